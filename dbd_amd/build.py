@@ -1,0 +1,81 @@
+"""Build libdbd.so (gfx950 HIP kernels + C++ engine) and the CLI drop-ins, in-tree.
+
+Outputs:
+    dbd_amd/libdbd.so                 the C-ABI library (include/dbd.h)
+    dbd_amd/bin/{compute,d2h,SSpMM,debug}   reference-compatible command lines
+
+Run: python -m dbd_amd.build   (or python dbd_amd/build.py)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+BIN = os.path.join(HERE, "bin")
+LIB = os.path.join(HERE, "libdbd.so")
+
+ARCH = os.environ.get("DBD_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+    # packed FP32 ops disabled on purpose (hipcc flag only)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-pass-failed", "-I", os.path.join(ROOT, "include"),
+]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{msg}")
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    objs = {
+        "kernels.o": [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip")],
+        "engine.o": [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip")],
+        "host.o": [CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "host.cpp")],
+    }
+    jobs = []
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        for name, cmd in objs.items():
+            jobs.append(ex.submit(_run, cmd + ["-o", os.path.join(BUILD, name)]))
+        for j in jobs:
+            j.result()
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp,
+          *[os.path.join(BUILD, n) for n in objs], "-lpthread"])
+    os.replace(tmp, LIB)
+    modes = {"compute": ["-DDBD_CLI_MODE=0"], "d2h": ["-DDBD_CLI_MODE=1"],
+             "SSpMM": ["-DDBD_CLI_MODE=2"], "debug": ["-DDBD_CLI_MODE=2", "-DDBD_CLI_DEBUG"]}
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        futs = [ex.submit(_run, [CXX, *CXX_FLAGS, *defs, os.path.join(CSRC, "cli.cpp"), "-o",
+                                 os.path.join(BIN, name), "-L", HERE, "-ldbd",
+                                 "-Wl,-rpath,$ORIGIN/..", "-lpthread"])
+                for name, defs in modes.items()]
+        for f in futs:
+            f.result()
+    if verbose:
+        print(f"built {LIB} and {BIN}/{{compute,d2h,SSpMM,debug}} for {ARCH}")
+    return LIB
+
+
+if __name__ == "__main__":
+    try:
+        build()
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

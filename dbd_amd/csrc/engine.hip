@@ -1,0 +1,1120 @@
+// The DBD engine on one MI355X: planner, category dispatch, chunked output pipeline with
+// D2H into pinned host memory and parallel file writers, device two-phase API, backward,
+// batched hipGraph form and the autotuner. All exported symbols are declared in
+// include/dbd.h.
+//
+// Reference pipeline being replaced (Z/ = dbd/src of DBD_FULL.zip, S/ = for_small_datasets):
+//   divide_Blocks + get_UpperBound   Z/main.cpp:259-310, S/preProc.cu:142-156   -> plan()
+//   distribute_Workload dispatcher    Z/distributeWorkload.cpp:25-223           -> run()
+//   compute_GPU per block             S/computeGPU.cu:15-98                     -> numeric chunks
+//   write_File double buffers         Z/writeFile.cpp:13-154                    -> Writer
+// The reference dispatches one 512/4096-row block at a time to a GPU thread through
+// usleep-polled flags and synchronous cudaMemcpy. Here blocks are the unit of planning and
+// of output streaming, while kernels cover whole chunks of blocks per category, D2H runs on
+// its own stream, and writers pwrite chunks at their final file offsets.
+#include <fcntl.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace dbd;
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return set_error(DBD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                                      \
+  } while (0)
+
+#define CHK(expr)                        \
+  do {                                   \
+    dbd_status s_ = (expr);              \
+    if (s_ != DBD_OK) return s_;         \
+  } while (0)
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  dbd_status ensure(size_t n) {
+    if (n <= bytes && p) return DBD_OK;
+    release();
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      (void)hipGetLastError();
+      return set_error(DBD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    bytes = want;
+    return DBD_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  dbd_status ensure(size_t n) {
+    if (n <= bytes && p) return DBD_OK;
+    release();
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      (void)hipGetLastError();
+      return set_error(DBD_ERR_NOMEM, "hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    bytes = want;
+    return DBD_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+// ---- category tables -------------------------------------------------------------------
+struct Tables {
+  CatTable sym_ct, num_ct;
+  KernelShape sym_k[kMaxCats], num_k[kMaxCats];
+  int id = 0;
+};
+
+// Default table. Symbolic key: min(P_i, ncols) (a bound on nnz_i); numeric key: nnz_i.
+// Hash tables are sized so the load stays <= 1/2.
+Tables default_tables(int g_small, int g_big, double load) {
+  Tables t;
+  const int Ts[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
+  const int NTs[] = {64, 64, 64, 64, 128, 256, 512, 1024, 1024};
+  int n = 0;
+  t.sym_ct.lim[n] = 0;
+  t.sym_k[n++] = {0, 0, 0};
+  for (int i = 0; i < 9; ++i) {
+    t.sym_ct.lim[n] = (int64_t)(Ts[i] * load);
+    t.sym_k[n++] = {Ts[i], NTs[i], Ts[i] <= 1024 ? g_small : g_big};
+  }
+  t.sym_ct.lim[n] = INT64_MAX;
+  t.sym_k[n++] = {0, 1024, 16};  // long rows
+  t.sym_ct.n = n;
+  n = 0;
+  t.num_ct.lim[n] = 0;
+  t.num_k[n++] = {0, 0, 0};
+  for (int i = 0; i < 8; ++i) {
+    t.num_ct.lim[n] = (int64_t)(Ts[i] * load);
+    t.num_k[n++] = {Ts[i], NTs[i], Ts[i] <= 1024 ? g_small : g_big};
+  }
+  t.num_ct.lim[n] = INT64_MAX;
+  t.num_k[n++] = {0, 256, 16};
+  t.num_ct.n = n;
+  return t;
+}
+
+std::vector<Tables>& candidate_tables() {
+  static std::vector<Tables> c = [] {
+    std::vector<Tables> v;
+    int id = 0;
+    for (int gs : {16, 64})
+      for (int gb : {16, 64})
+        for (double load : {0.5, 0.25}) {
+          Tables t = default_tables(gs, gb, load);
+          t.id = id++;
+          v.push_back(t);
+        }
+    return v;
+  }();
+  return c;
+}
+
+// autotuner cache: problem class -> table id
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune_cache;
+
+// ---- writer pool for DBD_MODE_IO ---------------------------------------------------------
+struct WriteJob {
+  int slot;
+  hipEvent_t ready;
+  const void* idx;
+  const void* val;
+  int64_t n;
+  int64_t off;  // element offset of the chunk in C
+};
+
+struct Writer {
+  int fd_idx = -1, fd_val = -1;
+  size_t vsz = 8;
+  std::mutex mu;
+  std::condition_variable cv_job, cv_free;
+  std::deque<WriteJob> q;
+  std::vector<int> slot_busy;
+  std::vector<std::thread> th;
+  bool stop = false;
+  std::atomic<bool> failed{false};
+  std::string why;
+  double busy_s = 0;
+  double bytes = 0;
+
+  void start(int nthreads, int nslots) {
+    slot_busy.assign(nslots, 0);
+    stop = false;
+    failed = false;
+    for (int i = 0; i < nthreads; ++i) th.emplace_back([this] { loop(); });
+  }
+  void loop() {
+    for (;;) {
+      WriteJob j;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_job.wait(lk, [&] { return stop || !q.empty(); });
+        if (q.empty()) return;
+        j = q.front();
+        q.pop_front();
+      }
+      double t0 = now_s();
+      bool ok = hipEventSynchronize(j.ready) == hipSuccess;
+      ok = ok && pw(fd_idx, j.idx, 4 * (size_t)j.n, (off_t)(4 * j.off));
+      ok = ok && pw(fd_val, j.val, vsz * (size_t)j.n, (off_t)(vsz * (size_t)j.off));
+      double t1 = now_s();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        busy_s += t1 - t0;
+        bytes += (double)(4 + vsz) * (double)j.n;
+        if (!ok && !failed) {
+          failed = true;
+          why = strerror(errno);
+        }
+        slot_busy[j.slot] = 0;
+      }
+      cv_free.notify_all();
+    }
+  }
+  static bool pw(int fd, const void* buf, size_t n, off_t off) {
+    const char* p = (const char*)buf;
+    while (n > 0) {
+      ssize_t w = pwrite(fd, p, n > ((size_t)1 << 30) ? ((size_t)1 << 30) : n, off);
+      if (w <= 0) {
+        if (w < 0 && errno == EINTR) continue;
+        return false;
+      }
+      p += w;
+      n -= (size_t)w;
+      off += w;
+    }
+    return true;
+  }
+  void acquire(int slot) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv_free.wait(lk, [&] { return slot_busy[slot] == 0 || failed; });
+    slot_busy[slot] = 1;
+  }
+  void push(const WriteJob& j) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      q.push_back(j);
+    }
+    cv_job.notify_one();
+  }
+  void finish() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv_job.notify_all();
+    for (auto& t : th) t.join();
+    th.clear();
+  }
+};
+
+}  // namespace
+
+struct dbd_ctx {
+  dbd_opts o{};
+  hipStream_t st = nullptr;
+  bool own_st = false;
+  hipStream_t cst = nullptr;  // copy stream
+  int dev = 0;
+  int num_cu = 256;
+  size_t vsz = 8;
+
+  // operands
+  Mat A{}, B{};
+  bool ops = false, same = false;
+  DevBuf upA[3], upB[3];
+  int64_t a_bytes = 0, b_bytes = 0;
+  double h2d_s = 0;
+
+  // plan state
+  Tables tb;
+  bool tuned_set = false;
+  bool planned = false, symb = false;
+  int64_t nr = 0, nb = 0, nwg = 0;
+  DevBuf P, Pptr, nnz, Cptr, rows_sym, rows_num, cnt, offs_sym, offs_num, tmp, cursor, blk, err;
+  std::vector<int64_t> h_blk, h_blockP, h_blockC, h_offs_num, h_sym_tot;
+  dbd_plan_info info{};
+
+  // outputs
+  DevBuf out_idx[2], out_val[2];
+  HostBuf h_cptr, h_idx, h_val;      // whole host result (D2H retain)
+  std::vector<HostBuf> ring_idx, ring_val;
+  bool dev_result = false, host_result = false;
+  hipEvent_t ev_comp[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> ev_slot;
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
+
+  // batch graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void* g_key[3] = {nullptr, nullptr, nullptr};
+  DevBuf g_rng;
+};
+
+namespace {
+
+dbd_status ensure_events(dbd_ctx* c) {
+  for (int i = 0; i < 2; ++i) {
+    if (!c->ev_comp[i]) HIPCHK(hipEventCreateWithFlags(&c->ev_comp[i], hipEventDisableTiming));
+    if (!c->ev_copy[i]) HIPCHK(hipEventCreateWithFlags(&c->ev_copy[i], hipEventDisableTiming));
+  }
+  if (!c->ev_t0) HIPCHK(hipEventCreate(&c->ev_t0));
+  if (!c->ev_t1) HIPCHK(hipEventCreate(&c->ev_t1));
+  if (!c->ev_c0) HIPCHK(hipEventCreate(&c->ev_c0));
+  if (!c->ev_c1) HIPCHK(hipEventCreate(&c->ev_c1));
+  return DBD_OK;
+}
+
+dbd_status check_err(dbd_ctx* c, const char* phase) {
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (e != 0)
+    return set_error(DBD_ERR_INTERNAL, "%s: device consistency check failed (flags 0x%x)", phase, e);
+  return DBD_OK;
+}
+
+// Row analysis, int64 prefix, blocks, reference criteria, symbolic bins.
+dbd_status plan(dbd_ctx* c) {
+  if (!c->ops) return set_error(DBD_ERR_STATE, "dbd_plan: operands not set");
+  const int64_t nr = c->A.nrows;
+  c->nr = nr;
+  c->nwg = bin_tiles(nr);
+  const int BR = c->o.block_rows;
+  c->nb = nr == 0 ? 0 : (nr - 1) / BR + 1;
+  hipStream_t s = c->st;
+  CHK(c->P.ensure(8 * (size_t)(nr + 1)));
+  CHK(c->Pptr.ensure(8 * (size_t)(nr + 1)));
+  CHK(c->nnz.ensure(8 * (size_t)(nr + 1)));
+  CHK(c->Cptr.ensure(8 * (size_t)(nr + 1)));
+  CHK(c->rows_sym.ensure(4 * (size_t)(nr + 1)));
+  CHK(c->rows_num.ensure(4 * (size_t)(nr + 1)));
+  const size_t ncnt = (size_t)kMaxCats * (size_t)c->nwg + 1;
+  CHK(c->cnt.ensure(8 * ncnt));
+  CHK(c->offs_sym.ensure(8 * (ncnt + 1)));
+  CHK(c->offs_num.ensure(8 * (ncnt + 1)));
+  CHK(c->tmp.ensure(8 * (size_t)scan_tmp_elems((int64_t)ncnt + nr + 1)));
+  CHK(c->blk.ensure(8 * (size_t)(std::max(c->nb, (int64_t)kMaxCats) + 2)));
+  CHK(c->err.ensure(sizeof(int)));
+  launch_fill_i32(c->err.as<int32_t>(), 1, 0, s);
+
+  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), s);
+  launch_exclusive_scan(c->P.as<int64_t>(), c->Pptr.as<int64_t>(), nr, c->tmp.as<int64_t>(), s);
+  launch_gather_strided(c->Pptr.as<int64_t>(), nr, BR, c->nb, c->blk.as<int64_t>(), s);
+  c->h_blk.assign((size_t)c->nb + 1, 0);
+  HIPCHK(hipMemcpyAsync(c->h_blk.data(), c->blk.p, 8 * (size_t)(c->nb + 1), hipMemcpyDeviceToHost, s));
+
+  // symbolic bins
+  const CatTable& ct = c->tb.sym_ct;
+  const int64_t n_off = (int64_t)ct.n * c->nwg;
+  launch_bin_count(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->cnt.as<int64_t>(), s);
+  launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_sym.as<int64_t>(), n_off, c->tmp.as<int64_t>(), s);
+  launch_bin_scatter(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->offs_sym.as<int64_t>(), c->rows_sym.as<int32_t>(), s);
+  DevBuf& g = c->g_rng;
+  CHK(g.ensure(8 * (kMaxCats + 2)));
+  launch_gather_strided(c->offs_sym.as<int64_t>(), n_off, c->nwg, ct.n, g.as<int64_t>(), s);
+  c->h_sym_tot.assign((size_t)ct.n + 1, 0);
+  HIPCHK(hipMemcpyAsync(c->h_sym_tot.data(), g.p, 8 * (size_t)(ct.n + 1), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+
+  // block table + reference criteria
+  c->h_blockP.assign((size_t)c->nb, 0);
+  for (int64_t b = 0; b < c->nb; ++b) c->h_blockP[(size_t)b] = c->h_blk[(size_t)b + 1] - c->h_blk[(size_t)b];
+  dbd_plan_info& in = c->info;
+  memset(&in, 0, sizeof(in));
+  std::vector<int64_t> bp((size_t)c->nb + 1);
+  {
+    // criteria from block totals: reuse the host logic on per-block sums
+    int64_t total = 0, mx = 0;
+    for (int64_t v : c->h_blockP) {
+      total += v;
+      mx = std::max(mx, v);
+    }
+    in.products = total;
+    in.max_block_products = mx;
+    in.num_blocks = c->nb;
+    in.block_rows = BR;
+    if (c->nb > 0) {
+      const int64_t avg = (total / c->nb) / BR;
+      float sd = 0.0f;
+      for (int64_t v : c->h_blockP) {
+        const int64_t d = avg - v / BR;
+        sd += (float)(d * d);
+      }
+      sd /= (float)c->nb;
+      sd = sqrtf(sd);
+      in.criteria = (int64_t)(1.28f * sd) + avg;
+      for (int64_t v : c->h_blockP) {
+        switch (dbd_analyze_block(in.criteria, v / BR)) {
+          case 0: ++in.blocks_gpu_only; break;
+          case 1: ++in.blocks_cpu_pref; break;
+          default: ++in.blocks_any; break;
+        }
+      }
+    }
+  }
+  in.nrows = nr;
+  in.ncols = c->B.ncols;
+  in.nnz = -1;
+  in.n_sym_cats = ct.n;
+  in.n_num_cats = c->tb.num_ct.n;
+  for (int i = 0; i < ct.n; ++i) {
+    in.sym_rows[i] = c->h_sym_tot[(size_t)i + 1] - c->h_sym_tot[(size_t)i];
+    in.sym_limit[i] = ct.lim[i];
+  }
+  for (int i = 0; i < c->tb.num_ct.n; ++i) {
+    in.num_rows[i] = -1;
+    in.num_limit[i] = c->tb.num_ct.lim[i];
+  }
+  c->planned = true;
+  c->symb = false;
+  return DBD_OK;
+}
+
+dbd_status ensure_cursor(dbd_ctx* c) { return c->cursor.ensure(8 * (size_t)(c->A.nnz + 1)); }
+
+// Symbolic pass over all rows -> nnz_i -> Cptr (device, ctx-owned) -> numeric bins.
+dbd_status symbolic(dbd_ctx* c) {
+  if (!c->planned) return set_error(DBD_ERR_STATE, "symbolic before plan");
+  hipStream_t s = c->st;
+  const CatTable& ct = c->tb.sym_ct;
+  const int64_t* offs = c->offs_sym.as<int64_t>();
+  for (int k = 1; k < ct.n; ++k) {
+    const int64_t n = c->h_sym_tot[(size_t)k + 1] - c->h_sym_tot[(size_t)k];
+    if (n <= 0) continue;
+    if (c->tb.sym_k[k].T == 0) CHK(ensure_cursor(c));
+    launch_symbolic(c->tb.sym_k[k], c->rows_sym.as<int32_t>(), offs + (int64_t)k * c->nwg,
+                    offs + (int64_t)(k + 1) * c->nwg, n, 0, c->A, c->B, c->nnz.as<int64_t>(),
+                    c->cursor.as<int64_t>(), c->err.as<int>(), s);
+  }
+  launch_exclusive_scan(c->nnz.as<int64_t>(), c->Cptr.as<int64_t>(), c->nr, c->tmp.as<int64_t>(), s);
+  const int BR = c->o.block_rows;
+  launch_gather_strided(c->Cptr.as<int64_t>(), c->nr, BR, c->nb, c->blk.as<int64_t>(), s);
+  std::vector<int64_t> hb((size_t)c->nb + 1, 0);
+  HIPCHK(hipMemcpyAsync(hb.data(), c->blk.p, 8 * (size_t)(c->nb + 1), hipMemcpyDeviceToHost, s));
+  // numeric bins by exact nnz_i
+  const CatTable& nt = c->tb.num_ct;
+  const int64_t n_off = (int64_t)nt.n * c->nwg;
+  launch_bin_count(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s);
+  launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_num.as<int64_t>(), n_off, c->tmp.as<int64_t>(), s);
+  launch_bin_scatter(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->offs_num.as<int64_t>(), c->rows_num.as<int32_t>(),
+                     s);
+  c->h_offs_num.assign((size_t)n_off + 1, 0);
+  HIPCHK(hipMemcpyAsync(c->h_offs_num.data(), c->offs_num.p, 8 * (size_t)(n_off + 1), hipMemcpyDeviceToHost, s));
+  CHK(check_err(c, "symbolic"));  // synchronises the stream
+  c->h_blockC.assign((size_t)c->nb, 0);
+  for (int64_t b = 0; b < c->nb; ++b) c->h_blockC[(size_t)b] = hb[(size_t)b + 1] - hb[(size_t)b];
+  c->info.nnz = c->nb > 0 ? hb[(size_t)c->nb] : 0;
+  for (int k = 0; k < nt.n; ++k)
+    c->info.num_rows[k] = c->h_offs_num[(size_t)(k + 1) * c->nwg] - c->h_offs_num[(size_t)k * c->nwg];
+  c->symb = true;
+  return DBD_OK;
+}
+
+// Numeric kernels for rows of blocks [b0, b1) into Cidx/Cval at Cptr - cbase.
+dbd_status numeric_blocks(dbd_ctx* c, int64_t b0, int64_t b1, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
+                          void* Cval) {
+  const int BR = c->o.block_rows;
+  const int64_t r0 = b0 * BR, r1 = std::min(c->nr, b1 * BR);
+  const int64_t w0 = r0 / 256, w1 = (r1 + 255) / 256;  // bin tiles (BR is a multiple of 256)
+  const CatTable& nt = c->tb.num_ct;
+  const int64_t* offs = c->offs_num.as<int64_t>();
+  for (int k = 1; k < nt.n; ++k) {
+    const int64_t i0 = (int64_t)k * c->nwg + w0, i1 = (int64_t)k * c->nwg + w1;
+    const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
+    if (n <= 0) continue;
+    if (c->tb.num_k[k].T == 0) CHK(ensure_cursor(c));
+    launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
+                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st);
+  }
+  return DBD_OK;
+}
+
+struct Chunk {
+  int64_t b0, b1, e0, e1;  // blocks and element range of C
+};
+
+// Group consecutive blocks so each chunk's output stays within `cap` elements.
+dbd_status make_chunks(dbd_ctx* c, int64_t cap, std::vector<Chunk>& out) {
+  out.clear();
+  int64_t b = 0, e = 0;
+  while (b < c->nb) {
+    Chunk ch{b, b, e, e};
+    while (ch.b1 < c->nb) {
+      const int64_t add = c->h_blockC[(size_t)ch.b1];
+      if (ch.b1 > ch.b0 && (ch.e1 - ch.e0) + add > cap) break;
+      if (add > cap)
+        return set_error(DBD_ERR_BUDGET,
+                         "Not Enough Memory for Buffer: block %lld needs %lld output elements, chunk capacity %lld",
+                         (long long)ch.b1, (long long)add, (long long)cap);
+      ch.e1 += add;
+      ++ch.b1;
+    }
+    out.push_back(ch);
+    b = ch.b1;
+    e = ch.e1;
+  }
+  return DBD_OK;
+}
+
+int64_t fixed_device_bytes(dbd_ctx* c) {
+  int64_t s = c->a_bytes + (c->same ? 0 : c->b_bytes);
+  s += (int64_t)c->P.bytes + c->Pptr.bytes + c->nnz.bytes + c->Cptr.bytes + c->rows_sym.bytes + c->rows_num.bytes;
+  s += (int64_t)c->cnt.bytes + c->offs_sym.bytes + c->offs_num.bytes + c->tmp.bytes + c->cursor.bytes;
+  return s;
+}
+
+dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
+  clear_error();
+  if (!c->ops) return set_error(DBD_ERR_STATE, "dbd_run: operands not set");
+  if (mode < 0 || mode > 2) return set_error(DBD_ERR_INVALID, "dbd_run: mode must be 0, 1 or 2");
+  CHK(ensure_events(c));
+  dbd_timing t{};
+  t.h2d_s = c->h2d_s;
+  c->dev_result = c->host_result = false;
+  hipStream_t s = c->st;
+  const double t0 = now_s();
+  CHK(plan(c));
+  const double t1 = now_s();
+  t.preprocess_s = t1 - t0;
+  HIPCHK(hipEventRecord(c->ev_t0, s));
+  CHK(symbolic(c));
+  const int64_t nnzC = c->info.nnz;
+  const size_t esz = 4 + c->vsz;
+  const int64_t budget = c->o.device_budget_mib << 20;
+  const int64_t hbudget = c->o.host_budget_mib << 20;
+  const int64_t avail = budget - fixed_device_bytes(c);
+  if (avail <= 0)
+    return set_error(DBD_ERR_BUDGET, "Not Enough Memory for Dense Matrix for Result: device budget %lld MiB",
+                     (long long)c->o.device_budget_mib);
+  std::vector<Chunk> chunks;
+  const bool whole_dev = (mode == DBD_MODE_COMPUTE) && (int64_t)(nnzC * esz) <= avail;
+  int64_t cap;
+  if (whole_dev) {
+    cap = std::max<int64_t>(nnzC, 1);
+  } else {
+    cap = avail / (int64_t)(2 * esz);
+    if (mode != DBD_MODE_COMPUTE) {
+      const int hint = c->o.chunks_hint > 0 ? c->o.chunks_hint : 8;
+      cap = std::min<int64_t>(cap, std::max<int64_t>((nnzC + hint - 1) / hint, 1));
+    }
+    // largest block must fit
+    int64_t mxb = 0;
+    for (int64_t v : c->h_blockC) mxb = std::max(mxb, v);
+    cap = std::max(cap, std::min(mxb, avail / (int64_t)(2 * esz)));
+  }
+  CHK(make_chunks(c, cap, chunks));
+  int64_t maxch = 1;
+  for (auto& ch : chunks) maxch = std::max(maxch, ch.e1 - ch.e0);
+  const int nbuf = (whole_dev || mode == DBD_MODE_COMPUTE) ? 1 : 2;
+  for (int i = 0; i < nbuf; ++i) {
+    CHK(c->out_idx[i].ensure(4 * (size_t)maxch));
+    CHK(c->out_val[i].ensure(c->vsz * (size_t)maxch));
+  }
+  // host side
+  bool whole_host = false;
+  Writer wr;
+  int nslots = 0;
+  if (mode != DBD_MODE_COMPUTE) {
+    CHK(c->h_cptr.ensure(8 * (size_t)(c->nr + 1)));
+    HIPCHK(hipMemcpyAsync(c->h_cptr.p, c->Cptr.p, 8 * (size_t)(c->nr + 1), hipMemcpyDeviceToHost, c->st));
+    t.d2h_bytes += 8.0 * (double)(c->nr + 1);
+    whole_host = (mode == DBD_MODE_D2H) && c->o.retain_result && (int64_t)(nnzC * esz) <= hbudget;
+    if (whole_host) {
+      CHK(c->h_idx.ensure(4 * (size_t)std::max<int64_t>(nnzC, 1)));
+      CHK(c->h_val.ensure(c->vsz * (size_t)std::max<int64_t>(nnzC, 1)));
+    } else {
+      int want = (mode == DBD_MODE_IO) ? std::max(2, c->o.writer_threads + 1) : 2;
+      const int64_t per = maxch * (int64_t)esz;
+      if (per * 2 > hbudget)
+        return set_error(DBD_ERR_BUDGET, "Not Enough Memory for Buffer: host budget %lld MiB < two chunks of %lld B",
+                         (long long)c->o.host_budget_mib, (long long)per);
+      want = (int)std::min<int64_t>(want, hbudget / per);
+      nslots = want;
+      c->ring_idx.resize(nslots);
+      c->ring_val.resize(nslots);
+      for (int i = 0; i < nslots; ++i) {
+        CHK(c->ring_idx[i].ensure(4 * (size_t)maxch));
+        CHK(c->ring_val[i].ensure(c->vsz * (size_t)maxch));
+      }
+      while ((int)c->ev_slot.size() < nslots) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_slot.push_back(e);
+      }
+    }
+    if (mode == DBD_MODE_IO) {
+      std::string d = (c->o.out_dir && c->o.out_dir[0]) ? std::string(c->o.out_dir) + "/" : std::string();
+      std::string pi = d + "Idx.output", pv = d + "Val.output";
+      wr.fd_idx = open(pi.c_str(), O_CREAT | O_WRONLY | O_TRUNC, 0644);
+      wr.fd_val = open(pv.c_str(), O_CREAT | O_WRONLY | O_TRUNC, 0644);
+      if (wr.fd_idx < 0 || wr.fd_val < 0) {
+        if (wr.fd_idx >= 0) close(wr.fd_idx);
+        if (wr.fd_val >= 0) close(wr.fd_val);
+        return set_error(DBD_ERR_IO, "cannot open %s / %s: %s", pi.c_str(), pv.c_str(), strerror(errno));
+      }
+      wr.vsz = c->vsz;
+      if (ftruncate(wr.fd_idx, (off_t)(4 * nnzC)) != 0 || ftruncate(wr.fd_val, (off_t)(c->vsz * nnzC)) != 0) {
+        close(wr.fd_idx);
+        close(wr.fd_val);
+        return set_error(DBD_ERR_IO, "cannot size output files: %s", strerror(errno));
+      }
+      wr.start(c->o.writer_threads > 0 ? c->o.writer_threads : 4, nslots);
+    }
+  }
+  // chunk loop
+  double comp_ms = 0, copy_ms = 0;
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    const Chunk& ch = chunks[i];
+    const int bi = (int)(i % (size_t)nbuf);
+    if (mode != DBD_MODE_COMPUTE && i >= (size_t)nbuf) HIPCHK(hipStreamWaitEvent(s, c->ev_copy[bi], 0));
+    int32_t* di = c->out_idx[bi].as<int32_t>();
+    void* dvv = c->out_val[bi].p;
+    CHK(numeric_blocks(c, ch.b0, ch.b1, c->Cptr.as<int64_t>(), ch.e0, di, dvv));
+    if (mode == DBD_MODE_COMPUTE) continue;
+    HIPCHK(hipEventRecord(c->ev_comp[bi], s));
+    HIPCHK(hipStreamWaitEvent(c->cst, c->ev_comp[bi], 0));
+    const int64_t n = ch.e1 - ch.e0;
+    void *hi, *hv;
+    int slot = -1;
+    if (whole_host) {
+      hi = c->h_idx.as<char>() + 4 * ch.e0;
+      hv = c->h_val.as<char>() + c->vsz * ch.e0;
+    } else {
+      slot = (int)(i % (size_t)nslots);
+      if (mode == DBD_MODE_IO) {
+        wr.acquire(slot);
+        if (wr.failed) break;
+      }
+      hi = c->ring_idx[slot].p;
+      hv = c->ring_val[slot].p;
+    }
+    if (n > 0) {
+      HIPCHK(hipMemcpyAsync(hi, di, 4 * (size_t)n, hipMemcpyDeviceToHost, c->cst));
+      HIPCHK(hipMemcpyAsync(hv, dvv, c->vsz * (size_t)n, hipMemcpyDeviceToHost, c->cst));
+    }
+    t.d2h_bytes += (double)esz * (double)n;
+    HIPCHK(hipEventRecord(c->ev_copy[bi], c->cst));
+    if (mode == DBD_MODE_IO) {
+      HIPCHK(hipEventRecord(c->ev_slot[slot], c->cst));
+      wr.push({slot, c->ev_slot[slot], hi, hv, n, ch.e0});
+    }
+  }
+  HIPCHK(hipEventRecord(c->ev_t1, s));
+  HIPCHK(hipStreamSynchronize(c->cst));
+  HIPCHK(hipStreamSynchronize(s));
+  if (mode == DBD_MODE_IO) {
+    wr.finish();
+    close(wr.fd_idx);
+    close(wr.fd_val);
+    if (wr.failed) return set_error(DBD_ERR_IO, "Error during File Write: %s", wr.why.c_str());
+    // Ptr.output last, like write_Ptr (Z/writeFile.cpp:177-197)
+    const double w0 = now_s();
+    dbd_csr pc{};
+    pc.nrows = c->nr;
+    pc.ncols = c->B.ncols;
+    pc.nnz = 0;
+    pc.ptr = c->h_cptr.as<int64_t>();
+    pc.location = DBD_HOST;
+    pc.dtype = c->o.dtype;
+    std::string d = (c->o.out_dir && c->o.out_dir[0]) ? std::string(c->o.out_dir) + "/" : std::string();
+    std::string pp = d + "Ptr.output";
+    int fd = open(pp.c_str(), O_CREAT | O_WRONLY | O_TRUNC, 0644);
+    if (fd < 0) return set_error(DBD_ERR_IO, "cannot open %s", pp.c_str());
+    bool ok;
+    const bool p64 = c->o.ptr64_file || nnzC > INT32_MAX;
+    if (p64) {
+      ok = Writer::pw(fd, c->h_cptr.p, 8 * (size_t)(c->nr + 1), 0);
+    } else {
+      std::vector<int32_t> p32((size_t)c->nr + 1);
+      const int64_t* hp = c->h_cptr.as<int64_t>();
+      for (int64_t r = 0; r <= c->nr; ++r) p32[(size_t)r] = (int32_t)hp[r];
+      ok = Writer::pw(fd, p32.data(), 4 * p32.size(), 0);
+    }
+    ok = (close(fd) == 0) && ok;
+    if (!ok) return set_error(DBD_ERR_IO, "write to %s failed", pp.c_str());
+    t.write_s = wr.busy_s + (now_s() - w0);
+    t.io_bytes = wr.bytes + (p64 ? 8.0 : 4.0) * (double)(c->nr + 1);
+  }
+  CHK(check_err(c, "numeric"));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
+  comp_ms = ms;
+  (void)copy_ms;
+  const double t2 = now_s();
+  t.compute_gpu_s = comp_ms * 1e-3;
+  t.kernel_exec_s = t2 - t1;
+  t.total_s = t2 - t0;
+  t.chunks = (int64_t)chunks.size();
+  t.d2h_s = 0;
+  c->dev_result = whole_dev && c->o.retain_result && mode == DBD_MODE_COMPUTE;
+  c->host_result = whole_host;
+  if (tm) *tm = t;
+  return DBD_OK;
+}
+
+dbd_status upload(dbd_ctx* c, const dbd_csr* M, Mat& m, DevBuf* up, int64_t& bytes) {
+  if (M->dtype != c->o.dtype)
+    return set_error(DBD_ERR_INVALID, "operand dtype %d differs from the context dtype %d", M->dtype, c->o.dtype);
+  if (M->nrows < 0 || M->ncols < 0 || M->nnz < 0 || M->nrows >= INT32_MAX || M->ncols > INT32_MAX)
+    return set_error(DBD_ERR_OVERFLOW, "matrix dimensions must be below 2^31");
+  m.nrows = M->nrows;
+  m.ncols = M->ncols;
+  m.nnz = M->nnz;
+  const size_t vs = dtype_size(M->dtype);
+  bytes = (int64_t)(8 * (M->nrows + 1) + (4 + vs) * M->nnz);
+  if (M->location == DBD_DEVICE) {
+    m.ptr = M->ptr;
+    m.idx = M->idx;
+    m.val = M->val;
+    return DBD_OK;
+  }
+  // host view: validate structure, then upload
+  if (M->ptr[0] != 0 || M->ptr[M->nrows] != M->nnz)
+    return set_error(DBD_ERR_INVALID, "row pointer must start at 0 and end at nnz");
+  for (int64_t r = 0; r < M->nrows; ++r) {
+    if (M->ptr[r + 1] < M->ptr[r]) return set_error(DBD_ERR_INVALID, "row pointer decreases at row %lld", (long long)r);
+    for (int64_t i = M->ptr[r]; i < M->ptr[r + 1]; ++i) {
+      const int32_t col = M->idx[i];
+      if (col < 0 || col >= M->ncols)
+        return set_error(DBD_ERR_INVALID, "column %d out of range in row %lld", col, (long long)r);
+      if (i > M->ptr[r] && col <= M->idx[i - 1])
+        return set_error(DBD_ERR_INVALID, "columns of row %lld are not strictly increasing", (long long)r);
+    }
+  }
+  CHK(up[0].ensure(8 * (size_t)(M->nrows + 1)));
+  CHK(up[1].ensure(4 * (size_t)std::max<int64_t>(M->nnz, 1)));
+  CHK(up[2].ensure(vs * (size_t)std::max<int64_t>(M->nnz, 1)));
+  HIPCHK(hipMemcpyAsync(up[0].p, M->ptr, 8 * (size_t)(M->nrows + 1), hipMemcpyHostToDevice, c->st));
+  if (M->nnz > 0) {
+    HIPCHK(hipMemcpyAsync(up[1].p, M->idx, 4 * (size_t)M->nnz, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(up[2].p, M->val, vs * (size_t)M->nnz, hipMemcpyHostToDevice, c->st));
+  }
+  m.ptr = up[0].as<int64_t>();
+  m.idx = up[1].as<int32_t>();
+  m.val = up[2].p;
+  return DBD_OK;
+}
+
+std::string problem_class(dbd_ctx* c) {
+  // coarse features: dtype, log2 of products per row, log2 ncols, log2 of nnz per B row
+  auto lg = [](double x) { return x <= 1 ? 0 : (int)std::floor(std::log2(x)); };
+  char buf[128];
+  const double ppr = c->nr > 0 ? (double)c->info.products / (double)c->nr : 0;
+  const double bpr = c->B.nrows > 0 ? (double)c->B.nnz / (double)c->B.nrows : 0;
+  snprintf(buf, sizeof(buf), "dt%d_ppr%d_nc%d_bpr%d", c->o.dtype, lg(ppr), lg((double)c->B.ncols), lg(bpr));
+  return buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+dbd_status dbd_create(const dbd_opts* opts, dbd_ctx** out) {
+  clear_error();
+  if (!out) return set_error(DBD_ERR_INVALID, "dbd_create: out is NULL");
+  *out = nullptr;
+  dbd_opts o;
+  dbd_default_opts(&o);
+  if (opts) {
+    o = *opts;
+    if (o.block_rows == 0) o.block_rows = 4096;
+    if (o.host_budget_mib == 0) o.host_budget_mib = 28 << 10;
+    if (o.device_budget_mib == 0) o.device_budget_mib = 22 << 10;
+    if (o.writer_threads == 0) o.writer_threads = 4;
+    if (o.chunks_hint == 0) o.chunks_hint = 8;
+  }
+  if (o.block_rows <= 0 || o.block_rows % 256 != 0)
+    return set_error(DBD_ERR_INVALID, "block_rows must be a positive multiple of 256 (got %d)", o.block_rows);
+  if (o.dtype != DBD_F32 && o.dtype != DBD_F64) return set_error(DBD_ERR_INVALID, "dtype must be DBD_F32 or DBD_F64");
+  if (o.host_budget_mib < 0 || o.device_budget_mib < 0) return set_error(DBD_ERR_INVALID, "negative budget");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return set_error(DBD_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  }
+  dbd_ctx* c = new dbd_ctx();
+  c->o = o;
+  c->vsz = dtype_size(o.dtype);
+  c->tb = default_tables(16, 64, 0.5);
+  if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_error(DBD_ERR_HIP, "stream creation failed");
+  }
+  c->own_st = true;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->dev) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  *out = c;
+  return DBD_OK;
+}
+
+dbd_status dbd_destroy(dbd_ctx* c) {
+  if (!c) return DBD_OK;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->cst) (void)hipStreamSynchronize(c->cst);
+  for (auto* b : {&c->P, &c->Pptr, &c->nnz, &c->Cptr, &c->rows_sym, &c->rows_num, &c->cnt, &c->offs_sym, &c->offs_num,
+                  &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng})
+    b->release();
+  for (int i = 0; i < 3; ++i) {
+    c->upA[i].release();
+    c->upB[i].release();
+  }
+  for (int i = 0; i < 2; ++i) {
+    c->out_idx[i].release();
+    c->out_val[i].release();
+    if (c->ev_comp[i]) (void)hipEventDestroy(c->ev_comp[i]);
+    if (c->ev_copy[i]) (void)hipEventDestroy(c->ev_copy[i]);
+  }
+  for (auto e : c->ev_slot) (void)hipEventDestroy(e);
+  for (auto e : {c->ev_t0, c->ev_t1, c->ev_c0, c->ev_c1})
+    if (e) (void)hipEventDestroy(e);
+  c->h_cptr.release();
+  c->h_idx.release();
+  c->h_val.release();
+  for (auto& b : c->ring_idx) b.release();
+  for (auto& b : c->ring_val) b.release();
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->own_st && c->st) (void)hipStreamDestroy(c->st);
+  if (c->cst) (void)hipStreamDestroy(c->cst);
+  delete c;
+  return DBD_OK;
+}
+
+dbd_status dbd_set_stream(dbd_ctx* c, void* stream) {
+  clear_error();
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (c->own_st) (void)hipStreamDestroy(c->st);
+  if (stream) {
+    c->st = (hipStream_t)stream;
+    c->own_st = false;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    c->own_st = true;
+  }
+  return DBD_OK;
+}
+
+dbd_status dbd_set_operands(dbd_ctx* c, const dbd_csr* A, const dbd_csr* B) {
+  clear_error();
+  if (!c || !A) return set_error(DBD_ERR_INVALID, "dbd_set_operands: null argument");
+  if (!B) B = A;
+  if (A->ncols != B->nrows)
+    return set_error(DBD_ERR_INVALID, "inner dimensions differ: A is %lld x %lld, B is %lld x %lld",
+                     (long long)A->nrows, (long long)A->ncols, (long long)B->nrows, (long long)B->ncols);
+  c->ops = false;
+  c->planned = c->symb = false;
+  c->dev_result = c->host_result = false;
+  const double t0 = now_s();
+  CHK(upload(c, A, c->A, c->upA, c->a_bytes));
+  c->same = (A == B) || (A->ptr == B->ptr && A->idx == B->idx && A->val == B->val);
+  if (c->same) {
+    c->B = c->A;
+    c->b_bytes = c->a_bytes;
+  } else {
+    CHK(upload(c, B, c->B, c->upB, c->b_bytes));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->h2d_s = (A->location == DBD_HOST || B->location == DBD_HOST) ? now_s() - t0 : 0.0;
+  c->ops = true;
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    (void)hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return DBD_OK;
+}
+
+dbd_status dbd_plan(dbd_ctx* c, dbd_plan_info* info) {
+  clear_error();
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  CHK(plan(c));
+  if (info) *info = c->info;
+  return DBD_OK;
+}
+
+dbd_status dbd_get_plan_info(dbd_ctx* c, dbd_plan_info* info) {
+  if (!c || !info) return set_error(DBD_ERR_INVALID, "null argument");
+  if (!c->planned) return set_error(DBD_ERR_STATE, "not planned");
+  *info = c->info;
+  return DBD_OK;
+}
+
+dbd_status dbd_get_row_products(dbd_ctx* c, int64_t* hP) {
+  clear_error();
+  if (!c || !hP) return set_error(DBD_ERR_INVALID, "null argument");
+  if (!c->planned) return set_error(DBD_ERR_STATE, "not planned");
+  HIPCHK(hipMemcpyAsync(hP, c->P.p, 8 * (size_t)c->nr, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return DBD_OK;
+}
+
+dbd_status dbd_get_blocks(dbd_ctx* c, int64_t* bptr, int64_t* bprod) {
+  clear_error();
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  if (!c->planned) return set_error(DBD_ERR_STATE, "not planned");
+  for (int64_t b = 0; b <= c->nb; ++b)
+    if (bptr) bptr[b] = std::min<int64_t>(b * c->o.block_rows, c->nr);
+  for (int64_t b = 0; b < c->nb; ++b)
+    if (bprod) bprod[b] = c->h_blockP[(size_t)b];
+  return DBD_OK;
+}
+
+dbd_status dbd_run(dbd_ctx* c, int32_t mode, dbd_timing* tm) {
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  return run(c, mode, tm);
+}
+
+dbd_status dbd_result(dbd_ctx* c, dbd_csr* C) {
+  clear_error();
+  if (!c || !C) return set_error(DBD_ERR_INVALID, "null argument");
+  memset(C, 0, sizeof(*C));
+  C->nrows = c->nr;
+  C->ncols = c->B.ncols;
+  C->nnz = c->info.nnz;
+  C->dtype = c->o.dtype;
+  if (c->dev_result) {
+    C->ptr = c->Cptr.as<int64_t>();
+    C->idx = c->out_idx[0].as<int32_t>();
+    C->val = c->out_val[0].p;
+    C->location = DBD_DEVICE;
+    return DBD_OK;
+  }
+  if (c->host_result) {
+    C->ptr = c->h_cptr.as<int64_t>();
+    C->idx = c->h_idx.as<int32_t>();
+    C->val = c->h_val.p;
+    C->location = DBD_HOST;
+    return DBD_OK;
+  }
+  return set_error(DBD_ERR_STATE, "no retained result (run COMPUTE or D2H with retain_result=1 within budget)");
+}
+
+dbd_status dbd_symbolic(dbd_ctx* c, int64_t* d_Cptr, int64_t* nnz_out) {
+  clear_error();
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  if (!c->planned) CHK(plan(c));
+  CHK(symbolic(c));
+  if (d_Cptr)
+    HIPCHK(hipMemcpyAsync(d_Cptr, c->Cptr.p, 8 * (size_t)(c->nr + 1), hipMemcpyDeviceToDevice, c->st));
+  if (nnz_out) *nnz_out = c->info.nnz;
+  return DBD_OK;
+}
+
+dbd_status dbd_numeric(dbd_ctx* c, const int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval) {
+  clear_error();
+  if (!c || !d_Cptr || (c->info.nnz > 0 && (!d_Cidx || !d_Cval))) return set_error(DBD_ERR_INVALID, "null argument");
+  if (!c->symb) return set_error(DBD_ERR_STATE, "dbd_numeric before dbd_symbolic");
+  launch_fill_i32(c->err.as<int32_t>(), 1, 0, c->st);
+  CHK(numeric_blocks(c, 0, c->nb, d_Cptr, 0, d_Cidx, d_Cval));
+  return check_err(c, "numeric");
+}
+
+dbd_status dbd_backward(dbd_ctx* c, const int64_t* d_Cptr, const int32_t* d_Cidx, const void* d_dC, void* d_dA,
+                        void* d_dB) {
+  clear_error();
+  if (!c || !d_Cptr || !d_Cidx || !d_dC) return set_error(DBD_ERR_INVALID, "null argument");
+  if (!c->symb) return set_error(DBD_ERR_STATE, "dbd_backward needs the forward plan (run dbd_symbolic first)");
+  hipStream_t s = c->st;
+  launch_fill_i32(c->err.as<int32_t>(), 1, 0, s);
+  if (d_dA) launch_fill_bytes0(d_dA, (int64_t)c->vsz * c->A.nnz, s);
+  const CatTable& nt = c->tb.num_ct;
+  const int64_t* offs = c->offs_num.as<int64_t>();
+  for (int k = 1; k < nt.n; ++k) {
+    const int64_t i0 = (int64_t)k * c->nwg, i1 = (int64_t)(k + 1) * c->nwg;
+    const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
+    if (n <= 0) continue;
+    if (c->tb.num_k[k].T == 0) CHK(ensure_cursor(c));
+    launch_backward(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
+                    d_Cptr, d_Cidx, d_dC, d_dA, d_dB, c->cursor.as<int64_t>(), c->err.as<int>(), s);
+  }
+  return check_err(c, "backward");
+}
+
+}  // extern "C"
+
+// ---- batched form ----------------------------------------------------------------------
+static int persistent_grid(dbd_ctx* c, const KernelShape& k) {
+  if (k.T == 0) return c->num_cu * 2;
+  return c->num_cu * (k.NT <= 64 ? 8 : 2);
+}
+
+// The whole pipeline (row analysis, bins, symbolic, scan, bins, numeric) on stream s with
+// persistent category kernels; no host synchronisation, so it can be captured.
+static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval) {
+  const int64_t nr = c->nr;
+  launch_fill_i32(c->err.as<int32_t>(), 1, 0, s);
+  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), s);
+  const CatTable& ct = c->tb.sym_ct;
+  launch_bin_count(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->cnt.as<int64_t>(), s);
+  launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_sym.as<int64_t>(), (int64_t)ct.n * c->nwg, c->tmp.as<int64_t>(), s);
+  launch_bin_scatter(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->offs_sym.as<int64_t>(), c->rows_sym.as<int32_t>(), s);
+  const int64_t* so = c->offs_sym.as<int64_t>();
+  for (int k = 1; k < ct.n; ++k)
+    launch_symbolic(c->tb.sym_k[k], c->rows_sym.as<int32_t>(), so + (int64_t)k * c->nwg, so + (int64_t)(k + 1) * c->nwg,
+                    0, persistent_grid(c, c->tb.sym_k[k]), c->A, c->B, c->nnz.as<int64_t>(), c->cursor.as<int64_t>(),
+                    c->err.as<int>(), s);
+  launch_exclusive_scan(c->nnz.as<int64_t>(), d_Cptr, nr, c->tmp.as<int64_t>(), s);
+  const CatTable& nt = c->tb.num_ct;
+  launch_bin_count(c->nnz.as<int64_t>(), nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s);
+  launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_num.as<int64_t>(), (int64_t)nt.n * c->nwg, c->tmp.as<int64_t>(), s);
+  launch_bin_scatter(c->nnz.as<int64_t>(), nr, INT64_MAX, nt, c->offs_num.as<int64_t>(), c->rows_num.as<int32_t>(), s);
+  const int64_t* no = c->offs_num.as<int64_t>();
+  for (int k = 1; k < nt.n; ++k)
+    launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), no + (int64_t)k * c->nwg,
+                   no + (int64_t)(k + 1) * c->nwg, 0, persistent_grid(c, c->tb.num_k[k]), c->A, c->B, d_Cptr, 0,
+                   d_Cidx, d_Cval, c->cursor.as<int64_t>(), c->err.as<int>(), s);
+  return DBD_OK;
+}
+
+extern "C" {
+
+dbd_status dbd_batch_prepare(dbd_ctx* c, const dbd_csr* A_cat, const dbd_csr* B_cat, int64_t* nnz_bound) {
+  clear_error();
+  if (!c || !A_cat) return set_error(DBD_ERR_INVALID, "null argument");
+  CHK(dbd_set_operands(c, A_cat, B_cat));
+  CHK(plan(c));
+  CHK(ensure_cursor(c));
+  if (nnz_bound) *nnz_bound = c->info.products;
+  return DBD_OK;
+}
+
+dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval) {
+  clear_error();
+  if (!c || !d_Cptr || !d_Cidx || !d_Cval) return set_error(DBD_ERR_INVALID, "null argument");
+  if (!c->planned) return set_error(DBD_ERR_STATE, "dbd_batch_launch before dbd_batch_prepare");
+  const void* key[3] = {d_Cptr, d_Cidx, d_Cval};
+  if (!c->gexec || memcmp(key, c->g_key, sizeof(key)) != 0) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+    dbd_status st = batch_enqueue(c, c->st, d_Cptr, d_Cidx, d_Cval);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (st != DBD_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess || !g) return set_error(DBD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    memcpy(c->g_key, key, sizeof(key));
+  }
+  HIPCHK(hipGraphLaunch(c->gexec, c->st));
+  CHK(check_err(c, "batch"));
+  c->symb = false;
+  return DBD_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+dbd_status dbd_autotune(dbd_ctx* c, int32_t* chosen) {
+  clear_error();
+  if (!c || !c->ops) return set_error(DBD_ERR_STATE, "dbd_autotune: operands not set");
+  CHK(ensure_events(c));
+  CHK(plan(c));
+  const std::string key = problem_class(c);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      c->tb = candidate_tables()[(size_t)it->second];
+      if (chosen) *chosen = it->second;
+      return DBD_OK;
+    }
+  }
+  // time symbolic + numeric for each candidate on the real operands
+  int best = 0;
+  double best_ms = 1e30;
+  DevBuf ci, cv;
+  for (const Tables& cand : candidate_tables()) {
+    c->tb = cand;
+    CHK(plan(c));
+    double tot = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIPCHK(hipEventRecord(c->ev_t0, c->st));
+      CHK(symbolic(c));
+      CHK(ci.ensure(4 * (size_t)std::max<int64_t>(c->info.nnz, 1)));
+      CHK(cv.ensure(c->vsz * (size_t)std::max<int64_t>(c->info.nnz, 1)));
+      CHK(numeric_blocks(c, 0, c->nb, c->Cptr.as<int64_t>(), 0, ci.as<int32_t>(), cv.p));
+      HIPCHK(hipEventRecord(c->ev_t1, c->st));
+      HIPCHK(hipEventSynchronize(c->ev_t1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
+      if (rep > 0) tot += ms;
+    }
+    CHK(check_err(c, "autotune"));
+    if (tot < best_ms) {
+      best_ms = tot;
+      best = cand.id;
+    }
+  }
+  ci.release();
+  cv.release();
+  c->tb = candidate_tables()[(size_t)best];
+  CHK(plan(c));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  if (chosen) *chosen = best;
+  return DBD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Host-side launchers for the gfx950 kernels in kernels.hip.
+//
+// Every row kernel is grid-stride over a "bin" (list of row ids), and reads the bin's
+// [begin, end) from device memory (two int64 slots). The plain pipeline launches exactly as
+// many workgroups as the bin needs; the batched/graph pipeline launches a fixed persistent
+// grid that drains the same ranges, so nothing depends on host-side counts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dbd {
+
+constexpr int kMaxCats = 16;
+
+// Category thresholds: row key k -> smallest c with k <= lim[c] (c = 0 is "k == 0"),
+// else the last category (the long-row path).
+struct CatTable {
+  int64_t lim[kMaxCats];
+  int32_t n;  // number of categories including 0 and the long category
+};
+
+struct Mat {
+  const int64_t* ptr;
+  const int32_t* idx;
+  const void* val;
+  int64_t nrows, ncols, nnz;
+};
+
+// One category's kernel shape, chosen by the planner/autotuner.
+struct KernelShape {
+  int T;   // LDS hash slots (power of two); 0 = long-row window kernel
+  int NT;  // threads per row group
+  int G;   // lanes cooperating on one A entry (B row)
+};
+
+// ---- generic device utilities ----
+void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void launch_fill_bytes0(void* p, int64_t nbytes, hipStream_t s);
+// Exclusive scan of n int64 (in may equal out); out[n] = total. tmp needs scan_tmp_elems(n).
+int64_t scan_tmp_elems(int64_t n);
+void launch_exclusive_scan(const int64_t* in, int64_t* out, int64_t n, int64_t* tmp, hipStream_t s);
+// out[b] = ptr[min(b*stride, n)] for b in [0, nb]
+void launch_gather_strided(const int64_t* ptr, int64_t n, int64_t stride, int64_t nb, int64_t* out, hipStream_t s);
+
+// ---- planning ----
+// P[i] = sum_{k in A_i} len(B_k); nnz[i] = 0 where P[i] == 0.
+void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, hipStream_t s);
+// Stable counting sort of rows [0, n) by category of min(key[i], cap).
+// counts/offs have ncat * nwg (+1) int64 where nwg = bin_tiles(n).
+int64_t bin_tiles(int64_t n);
+void launch_bin_count(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, int64_t* counts,
+                      hipStream_t s);
+void launch_bin_scatter(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, const int64_t* offs,
+                        int32_t* rows, hipStream_t s);
+
+// ---- symbolic / numeric / backward per category ----
+// grid == 0 -> exact grid from nrows_hint; else a persistent grid of that many workgroups.
+int symbolic_groups_per_wg(const KernelShape& k);
+void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
+                     int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
+                     int* err, hipStream_t s);
+void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
+                    const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
+                    const int64_t* Cptr, int64_t cbase, int32_t* Cidx, void* Cval, int64_t* cursor, int* err,
+                    hipStream_t s);
+void launch_backward(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
+                     const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
+                     const int64_t* Cptr, const int32_t* Cidx, const void* dC, void* dA, void* dB, int64_t* cursor,
+                     int* err, hipStream_t s);
+// Is this (T, NT, G) compiled? (the autotuner only proposes compiled shapes)
+bool shape_available(int phase, const KernelShape& k);  // phase 0 symbolic, 1 numeric, 2 backward
+
+}  // namespace dbd

@@ -1,0 +1,197 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same inputs.
+Structure bit-exact, values within rtol * sum|a b| (1e-12 FP64, 1e-5 FP32)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.io
+import scipy.sparse as sp
+
+import dbd_amd
+import oracle
+from helpers import check_product, compare, gpu_spgemm, reference
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def csr_sp(m, dtype=np.float64):
+    return dbd_amd.CSR.from_scipy(sp.csr_matrix(m), dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_kat_bhsparse(gpu_ok, dtype):
+    kat = json.load(open(os.path.join(GOLD, "bhsparse_kat.json")))
+    A = dbd_amd.CSR(np.array(kat["A"]["ptr"]), np.array(kat["A"]["idx"]), np.array(kat["A"]["val"], dtype),
+                    kat["A"]["shape"])
+    B = dbd_amd.CSR(np.array(kat["B"]["ptr"]), np.array(kat["B"]["idx"]), np.array(kat["B"]["val"], dtype),
+                    kat["B"]["shape"])
+    C, t, info = gpu_spgemm(A, B, block_rows=256)
+    assert C.ptr.tolist() == kat["C"]["ptr"]
+    assert C.idx.tolist() == kat["C"]["idx"]
+    assert C.val.tolist() == kat["C"]["val"]
+    assert info["products"] == kat["products"] and info["nnz"] == 6
+
+
+def test_cage4_and_laplacian(gpu_ok):
+    for name in ("cage4.mtx", "laplacian_5pt_10x10.mtx"):
+        A = csr_sp(scipy.io.mmread(os.path.join(GOLD, name)))
+        check_product(A, what=name, block_rows=256)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape,density,seed", [((300, 200, 250), 0.02, 1), ((1000, 1000, 1000), 0.01, 2),
+                                                ((64, 5000, 3000), 0.01, 3), ((2000, 300, 4000), 0.05, 4)])
+def test_random(gpu_ok, dtype, shape, density, seed):
+    m, k, n = shape
+    rng = np.random.default_rng(seed)
+    A = csr_sp(sp.random(m, k, density=density, random_state=seed, format="csr",
+                         data_rvs=lambda s: rng.standard_normal(s)), dtype)
+    B = csr_sp(sp.random(k, n, density=density, random_state=seed + 7, format="csr",
+                         data_rvs=lambda s: rng.standard_normal(s)), dtype)
+    check_product(A, B, what=f"random{shape}", block_rows=256)
+
+
+def test_empty_and_degenerate(gpu_ok):
+    # all-empty A
+    A = dbd_amd.CSR(np.zeros(11, np.int64), np.zeros(0, np.int32), np.zeros(0), (10, 10))
+    C, t, info = gpu_spgemm(A, block_rows=256)
+    assert C.nnz == 0 and C.ptr.tolist() == [0] * 11
+    # A non-empty but B rows empty -> all P_i = 0
+    A = csr_sp(sp.eye(300, 40, format="csr"))
+    B = dbd_amd.CSR(np.zeros(41, np.int64), np.zeros(0, np.int32), np.zeros(0), (40, 70))
+    C, t, info = gpu_spgemm(A, B, block_rows=256)
+    assert C.nnz == 0 and info["products"] == 0
+    # one row, one column
+    A = csr_sp(sp.csr_matrix(np.array([[2.0]])))
+    C, _, _ = gpu_spgemm(A, block_rows=256)
+    assert C.val.tolist() == [4.0]
+    # ragged: every other row empty, rectangular
+    M = sp.random(777, 333, density=0.03, random_state=5, format="lil")
+    M[::2, :] = 0
+    A = csr_sp(M.tocsr())
+    B = csr_sp(sp.random(333, 999, density=0.02, random_state=6, format="csr"))
+    check_product(A, B, what="ragged", block_rows=256)
+
+
+def _skewed_matrix(n_rows, ncols, long_rows, seed):
+    """Mostly short rows plus a few very long ones, so every category (tiny hash tables up to
+    the LDS window path) gets rows."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(n_rows):
+        if i in long_rows:
+            d = long_rows[i]
+        else:
+            d = int(min(ncols, rng.zipf(1.6)))
+        c = rng.choice(ncols, size=min(d, ncols), replace=False)
+        rows.append(np.full(c.size, i))
+        cols.append(c)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    v = rng.standard_normal(r.size)
+    return sp.csr_matrix((v, (r, c)), shape=(n_rows, ncols))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_all_categories(gpu_ok, dtype):
+    n = 6000
+    long_rows = {17: 3000, 900: 5000, 4500: 1200, 5999: 40}
+    A = _skewed_matrix(n, n, long_rows, 11)
+    B = _skewed_matrix(n, n, {3: 2500, 4: 6000, 100: 800}, 12)
+    Ac, Bc = csr_sp(A, dtype), csr_sp(B, dtype)
+    C, t, info = check_product(Ac, Bc, what="all-categories", block_rows=512)
+    # every symbolic and numeric category that exists got rows, including the long path
+    assert info["sym_rows"][-1] > 0, info
+    assert info["num_rows"][-1] > 0, info
+    assert sum(1 for x in info["num_rows"][1:] if x > 0) >= 6, info
+
+
+def test_block_rows_do_not_change_result(gpu_ok):
+    A = csr_sp(_skewed_matrix(3000, 3000, {5: 900}, 21))
+    ref = reference(A)
+    for br in (256, 512, 4096):
+        C, _, info = gpu_spgemm(A, block_rows=br)
+        compare(C, ref, np.float64, f"block_rows={br}")
+        assert info["num_blocks"] == (3000 - 1) // br + 1
+
+
+def test_plan_matches_oracle(gpu_ok):
+    """Row products, blocks and the reference criteria from the device plan vs the oracle."""
+    A = csr_sp(_skewed_matrix(5000, 5000, {7: 2000}, 31))
+    with dbd_amd.Context(block_rows=512) as ctx:
+        ctx.set_operands(A)
+        info = ctx.plan()
+        P = ctx.row_products(5000)
+        bptr, bprod = ctx.blocks(info["num_blocks"])
+    Po = oracle.row_products(A.ptr, A.idx, A.ptr)
+    assert P.tolist() == Po.tolist()
+    bi = oracle.blocks(Po, 512)
+    assert bprod.tolist() == bi.tolist()
+    crit = oracle.criteria(bi, 512)
+    assert info["criteria"] == crit and info["products"] == int(Po.sum())
+    cats = [oracle.analyze_block(crit, int(b // 512)) for b in bi]
+    assert (info["blocks_gpu_only"], info["blocks_cpu_pref"], info["blocks_any"]) == (
+        cats.count(0), cats.count(1), cats.count(2))
+
+
+def test_web_like_full_size(gpu_ok):
+    """Web-like graph of 281,903 rows (Zipf out-degree, 70% of targets within +-2000 of the
+    row), C = A*A in FP64 with 4096-row blocks: the CPU-runnable configuration, full size."""
+    for s in (2.1, 1.66):
+        A = dbd_amd.gen_web(281903, zipf_s=s, seed=1)
+        check_product(A, what=f"web s={s}", block_rows=4096)
+
+
+def test_rmat_fp32_a_times_b(gpu_ok):
+    A = dbd_amd.gen_rmat(14, 16, seed=1, add_identity=True, row_normalize=True, dtype=np.float32)
+    B = dbd_amd.gen_rmat(14, 16, seed=2, dtype=np.float32)
+    check_product(A, B, what="rmat14 fp32", block_rows=512)
+
+
+def test_rmat_fp64_a_times_a(gpu_ok):
+    A = dbd_amd.gen_rmat(15, 16, seed=3)
+    check_product(A, what="rmat15 fp64", block_rows=512)
+
+
+def test_er_16_per_row_sampled_full_size(gpu_ok):
+    """Erdos-Renyi 2,000,000^2 with 16 nnz/row, C = A*B FP64, 512-row blocks, 32 GB device
+    budget (the benchmark configuration): full-size properties plus sampled rows vs oracle."""
+    n = 2_000_000
+    A = dbd_amd.gen_er(n, n, 16, seed=1)
+    B = dbd_amd.gen_er(n, n, 16, seed=2)
+    C, t, info = gpu_spgemm(A, B, block_rows=512, device_gib=32)
+    assert info["products"] == 512_000_000
+    assert C.nnz == info["nnz"] and C.ptr[-1] == C.nnz
+    # sorted, in range
+    d = np.diff(C.idx.astype(np.int64))
+    row_start = np.zeros(C.nnz, bool)
+    row_start[C.ptr[:-1][np.diff(C.ptr) > 0]] = True
+    assert np.all((d > 0) | row_start[1:])
+    assert C.idx.min() >= 0 and C.idx.max() < n
+    # sampled rows vs oracle
+    rng = np.random.default_rng(0)
+    sel = np.sort(rng.choice(n, 4000, replace=False)).astype(np.int32)
+    Sptr, Sidx, Sval = oracle.spgemm_rows(sel, A, B)
+    for j, i in enumerate(sel):
+        a, b = C.ptr[i], C.ptr[i + 1]
+        assert np.array_equal(C.idx[a:b], Sidx[Sptr[j]:Sptr[j + 1]]), f"row {i}"
+        np.testing.assert_allclose(C.val[a:b], Sval[Sptr[j]:Sptr[j + 1]], rtol=1e-12, atol=1e-12)
+    # linearity: C x = A (B x) for random x, over the whole product
+    x = rng.standard_normal(n)
+    Csp = sp.csr_matrix((C.val, C.idx, C.ptr), shape=(n, n))
+    Asp = sp.csr_matrix((A.val, A.idx, A.ptr), shape=(n, n))
+    Bsp = sp.csr_matrix((B.val, B.idx, B.ptr), shape=(n, n))
+    np.testing.assert_allclose(Csp @ x, Asp @ (Bsp @ x), rtol=1e-9, atol=1e-9)
+
+
+def test_autotuned_tables_give_identical_results(gpu_ok):
+    """Whatever table the autotuner picks, the result is the same (structure exact)."""
+    A = csr_sp(_skewed_matrix(4000, 4000, {1: 1500, 2: 3000}, 41))
+    ref = reference(A)
+    with dbd_amd.Context(block_rows=512, tune=True) as ctx:
+        ctx.set_operands(A)
+        chosen = ctx.autotune()
+        ctx.run("compute")
+        C = ctx.result()
+    compare(C, ref, np.float64, f"tuned table {chosen}")
