@@ -183,6 +183,63 @@ __device__ __forceinline__ int hash_find(const int* hk, int c) {
   return -1;
 }
 
+// Walk every product of one row for a group of NT >= 64 threads. Each wave of the group loads
+// up to 64 A entries at once (lane j: column k, value, B row start and length), then its
+// 64/G subgroups take entries through wave shuffles, U entries per step, issuing all U B-row
+// gathers before handing the products to f(col, val). B rows longer than G lanes continue in
+// a tail loop. Control flow around the shuffles is wave-uniform.
+//   f(p, c, v, va): p = position of the product's B entry, c = its column, v = A_ik * B_kj,
+//   va = A_ik. WITHV = false skips the value loads (symbolic pass).
+template <int NT, int G, int U, bool WITHV, typename V, typename F>
+__device__ __forceinline__ void for_row_products(int t, int64_t a0, int64_t a1, const int32_t* __restrict__ Aidx,
+                                                 const V* __restrict__ Aval, const int64_t* __restrict__ Bptr,
+                                                 const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F&& f) {
+  static_assert(NT >= 64 && 64 % G == 0, "wave-level walk");
+  constexpr int NG = 64 / G;
+  const int lane = t & 63, wv = t >> 6;
+  const int sg = lane / G, lg = lane % G;
+  for (int64_t base = a0 + 64 * wv; base < a1; base += NT) {
+    const int64_t j = base + lane;
+    const bool ok = j < a1;
+    const int k = ok ? Aidx[j] : 0;
+    const V va = (WITHV && ok) ? Aval[j] : V(0);
+    const int64_t b0 = ok ? Bptr[k] : 0;
+    const int len = ok ? (int)(Bptr[k + 1] - b0) : 0;
+    const int cnt = (int)min((int64_t)64, a1 - base);
+    for (int e0 = 0; e0 < cnt; e0 += NG * U) {
+      int64_t pb[U];
+      int pl[U];
+      V pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * NG + sg;
+        const int src = e < 64 ? e : 63;
+        pb[u] = __shfl(b0, src);
+        pl[u] = __shfl(len, src);
+        pv[u] = WITHV ? __shfl(va, src) : V(0);
+        if (e >= cnt) pl[u] = 0;
+      }
+      int c[U];
+      V v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool has = lg < pl[u];
+        c[u] = has ? Bidx[pb[u] + lg] : -1;
+        v[u] = (WITHV && has) ? pv[u] * Bval[pb[u] + lg] : V(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c[u] >= 0) f(pb[u] + lg, c[u], v[u], pv[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        for (int q = G + lg; q < pl[u]; q += G) {
+          const int64_t p = pb[u] + q;
+          f(p, Bidx[p], WITHV ? pv[u] * Bval[p] : V(0), pv[u]);
+        }
+    }
+  }
+}
+
 // Error bits written to the device error word.
 constexpr int kErrTableFull = 1;
 constexpr int kErrCountMismatch = 2;
@@ -429,15 +486,12 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
     gsync<NT>();
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
     int cnt = 0, bad = 0;
-    for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
-      const int k = Aidx[ka];
-      const int64_t b0 = Bptr[k], b1 = Bptr[k + 1];
-      for (int64_t p = b0 + (t % G); p < b1; p += G) {
-        const int r = hash_insert_key<T>(K, Bidx[p]);
-        cnt += (r > 0);
-        bad |= (r < 0);
-      }
-    }
+    for_row_products<NT, G, 4, false, float>(t, a0, a1, Aidx, (const float*)nullptr, Bptr, Bidx,
+                                             (const float*)nullptr, [&](int64_t, int c, float, float) {
+                                               const int r = hash_insert_key<T>(K, c);
+                                               cnt += (r > 0);
+                                               bad |= (r < 0);
+                                             });
     cnt = Group<NT>::reduce(cnt, OpAdd(), sc[g]);
     bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
     if (t == 0) {
@@ -559,18 +613,11 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_numeric_hash(
     gsync<NT>();
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
     int cmin = INT_MAX, cmax = -1, bad = 0;
-    for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
-      const int k = Aidx[ka];
-      const V va = Aval[ka];
-      const int64_t b0 = Bptr[k], b1 = Bptr[k + 1];
-      for (int64_t p = b0 + (t % G); p < b1; p += G) {
-        const int c = Bidx[p];
-        const V v = va * Bval[p];
-        cmin = min(cmin, c);
-        cmax = max(cmax, c);
-        bad |= !hash_insert_add<T>(K, H, c, v);
-      }
-    }
+    for_row_products<NT, G, 4, true, V>(t, a0, a1, Aidx, Aval, Bptr, Bidx, Bval, [&](int64_t, int c, V v, V) {
+      cmin = min(cmin, c);
+      cmax = max(cmax, c);
+      bad |= !hash_insert_add<T>(K, H, c, v);
+    });
     gsync<NT>();
     cmin = Group<NT>::reduce(cmin, OpMin(), sc[g]);
     cmax = Group<NT>::reduce(cmax, OpMax(), sc[g]);

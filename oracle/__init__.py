@@ -142,3 +142,33 @@ def backward(A, B, Cptr, Cidx, dC):
                                  Bptr.ctypes.data, Bidx.ctypes.data, Bval.ctypes.data, Cptr.ctypes.data,
                                  Cidx.ctypes.data, dC.ctypes.data, dA.ctypes.data, dB.ctypes.data, Bidx.shape[0])
     return dA[: Aidx.shape[0]], dB[: Bidx.shape[0]], int(miss)
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_cpu_spgemm(A, B=None, rows_in_block: int = 4096):
+    """C = A*B by the reference's own CPU executor (compute_CPU of Z/computeCPU.cpp, FP32,
+    |acc| < 1e-4 first touch, first-touch column order), built by `make -C oracle ref`.
+    Returns per-row lists (cols, vals) in the executor's order."""
+    Bm = B if B is not None else A
+    lb = C.CDLL(REF_LIB)
+    P_ = C.c_void_p
+    lb.dbdref_spgemm.argtypes = [C.c_int, C.c_int, C.c_int, P_, P_, P_, P_, P_, P_, C.c_int, P_, P_, P_, P_]
+    lb.dbdref_spgemm.restype = C.c_int
+    nr = int(A.shape[0])
+    Ap, Ai, Av = _c(A.ptr, np.int32), _c(A.idx, np.int32), _c(A.val, np.float32)
+    Bp, Bi, Bv = _c(Bm.ptr, np.int32), _c(Bm.idx, np.int32), _c(Bm.val, np.float32)
+    P = row_products(A.ptr, A.idx, Bm.ptr)
+    off = np.zeros(nr + 1, np.int64)
+    off[1:] = np.cumsum(P)
+    cnt = np.zeros(nr, np.int32)
+    oi = np.zeros(max(int(off[-1]), 1), np.int32)
+    ov = np.zeros(max(int(off[-1]), 1), np.float32)
+    rc = lb.dbdref_spgemm(nr, int(A.shape[1]), int(Bm.shape[1]), Ap.ctypes.data, Ai.ctypes.data, Av.ctypes.data,
+                          Bp.ctypes.data, Bi.ctypes.data, Bv.ctypes.data, rows_in_block, cnt.ctypes.data,
+                          off.ctypes.data, oi.ctypes.data, ov.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("reference executor failed")
+    return [(oi[off[r]:off[r] + cnt[r]].copy(), ov[off[r]:off[r] + cnt[r]].copy()) for r in range(nr)]

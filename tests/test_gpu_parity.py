@@ -75,7 +75,7 @@ def test_empty_and_degenerate(gpu_ok):
     check_product(A, B, what="ragged", block_rows=256)
 
 
-def _skewed_matrix(n_rows, ncols, long_rows, seed):
+def _skewed_matrix(n_rows, ncols, long_rows, seed, zipf=1.6):
     """Mostly short rows plus a few very long ones, so every category (tiny hash tables up to
     the LDS window path) gets rows."""
     rng = np.random.default_rng(seed)
@@ -84,7 +84,7 @@ def _skewed_matrix(n_rows, ncols, long_rows, seed):
         if i in long_rows:
             d = long_rows[i]
         else:
-            d = int(min(ncols, rng.zipf(1.6)))
+            d = int(min(ncols, rng.zipf(zipf)))
         c = rng.choice(ncols, size=min(d, ncols), replace=False)
         rows.append(np.full(c.size, i))
         cols.append(c)
@@ -95,16 +95,17 @@ def _skewed_matrix(n_rows, ncols, long_rows, seed):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_all_categories(gpu_ok, dtype):
-    n = 6000
+    """Row lengths from 1 to tens of thousands: every hash-table size and both window paths
+    (symbolic key min(P_i, ncols) above 8192, numeric nnz_i above 4096) get rows."""
+    n, ncols = 6000, 60000
     long_rows = {17: 3000, 900: 5000, 4500: 1200, 5999: 40}
     A = _skewed_matrix(n, n, long_rows, 11)
-    B = _skewed_matrix(n, n, {3: 2500, 4: 6000, 100: 800}, 12)
+    B = _skewed_matrix(n, ncols, {3: 2500, 4: 20000, 100: 800}, 12, zipf=1.4)
     Ac, Bc = csr_sp(A, dtype), csr_sp(B, dtype)
     C, t, info = check_product(Ac, Bc, what="all-categories", block_rows=512)
-    # every symbolic and numeric category that exists got rows, including the long path
-    assert info["sym_rows"][-1] > 0, info
-    assert info["num_rows"][-1] > 0, info
-    assert sum(1 for x in info["num_rows"][1:] if x > 0) >= 6, info
+    assert info["sym_rows"][-1] > 0, info["sym_rows"]
+    assert info["num_rows"][-1] > 0, info["num_rows"]
+    assert sum(1 for x in info["num_rows"][1:] if x > 0) >= 6, info["num_rows"]
 
 
 def test_block_rows_do_not_change_result(gpu_ok):
