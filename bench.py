@@ -95,6 +95,34 @@ def load_traffic():
         return None
 
 
+def profile_run():
+    """Kernels for the counter passes: numeric pass x3 (one launch each), symbolic pass x1,
+    then `y = x + 0` over 1 GiB of float32 x3 (reads 1 GiB, writes 1 GiB per launch)."""
+    import numpy as np
+    import torch
+
+    import dbd_amd
+    torch.cuda.set_device(0)
+    A = dbd_amd.gen_er(N, N, DEG, seed=1)
+    B = dbd_amd.gen_er(N, N, DEG, seed=2)
+    ctx = dbd_amd.Context(block_rows=BLOCK_ROWS, dtype=np.float64, device_gib=DEVICE_GIB)
+    ctx.set_operands(A, B)
+    ctx.plan()
+    Cptr = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+    nnz = ctx.symbolic(Cptr)
+    Cidx = torch.empty(nnz, dtype=torch.int32, device="cuda")
+    Cval = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        ctx.numeric(Cptr, Cidx, Cval)
+    torch.cuda.synchronize()
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_run": "done", "nnz_C": nnz, "copy_bytes_each_way": x.numel() * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +132,25 @@ def main():
     ap.add_argument("--io-dir", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=5)
+    ap.add_argument("--workload", default="er2m", choices=["er2m", "web", "rmat22", "batch", "fwdbwd"],
+                    help="er2m (default headline) or one of the other named configurations")
+    ap.add_argument("--batch-members", type=int, default=512)
+    ap.add_argument("--rmat-scale", type=int, default=22)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3 counter passes: 3 numeric passes of the headline "
+                         "workload, then 3 elementwise copies of 1 GiB (the known-size reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "er2m":
+        import bench_workloads
+        getattr(bench_workloads, "run_" + args.workload)(args)
+        return
+    if args.profile:
+        profile_run()
+        return
 
     import numpy as np
     import torch

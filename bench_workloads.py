@@ -1,0 +1,276 @@
+"""The other named configurations, run through `python bench.py --workload NAME`.
+
+    web     web-like 281,903^2 C = A*A, FP64, 4096-row blocks (literal Zipf 2.1 out-degree and
+            the nnz-matched exponent 1.66), all three windows, full-size oracle on the host
+    rmat22  R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05), C = A*A, FP64, 512-row blocks, 64 GB
+            device budget: kernel window over the whole product (streamed in chunks), D2H and
+            file windows on a bounded leading row range
+    batch   512 independent C_i = A_i * B_i (FP32; rows log-uniform in [1e3, 2e5], mean
+            nnz/row U[4, 64] with Zipf(1.8) row skew): one captured graph vs per-matrix plain runs
+    fwdbwd  R-MAT scale 18 A (row-normalised adjacency + self loops) times an independent R-MAT
+            B, FP32, through torch.autograd: forward vs forward + backward
+
+Each prints one JSON line on stdout.
+"""
+import json
+import os
+import shutil
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(msg):
+    print(f"[bench:{os.getpid()}] {msg}", file=sys.stderr, flush=True)
+
+
+def _threads():
+    t = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(t, os.cpu_count() or 1))
+
+
+def _timed(ctx, mode, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        ctx.run(mode)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(steps):
+        last = ctx.run(mode)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, last
+
+
+def _windows(A, B, block_rows, dtype, steps, warmup, modes, device_gib=32, host_gib=48, io_dir=None):
+    import dbd_amd
+    out = {}
+    ctx = dbd_amd.Context(block_rows=block_rows, dtype=dtype, device_gib=device_gib, host_gib=host_gib)
+    ctx.set_operands(A, B)
+    info = ctx.plan()
+    P = info["products"]
+    t, tm = _timed(ctx, "compute", steps, warmup)
+    info = ctx.plan_info()
+    out["compute"] = {"ms": round(t * 1e3, 3), "gflops": round(2 * P / t / 1e9, 3), "chunks": tm["chunks"]}
+    if "d2h" in modes:
+        t, tm = _timed(ctx, "d2h", max(1, min(steps, 3)), 1)
+        out["d2h"] = {"ms": round(t * 1e3, 3), "gflops": round(2 * P / t / 1e9, 3), "d2h_bytes": tm["d2h_bytes"]}
+    ctx.close()
+    if "io" in modes:
+        d = io_dir or tempfile.mkdtemp(prefix="dbd_io_", dir="/tmp")
+        c2 = dbd_amd.Context(block_rows=block_rows, dtype=dtype, device_gib=device_gib, host_gib=host_gib,
+                             out_dir=d, writer_threads=8, chunks_hint=16)
+        c2.set_operands(A, B)
+        c2.run("io")
+        t0 = time.perf_counter()
+        tm = c2.run("io")
+        t = time.perf_counter() - t0
+        out["io"] = {"ms": round(t * 1e3, 3), "gflops": round(2 * P / t / 1e9, 3), "io_bytes": tm["io_bytes"],
+                     "write_s": round(tm["write_s"], 3)}
+        c2.close()
+        shutil.rmtree(d, ignore_errors=True)
+    return P, info, out
+
+
+def _cpu_windows(A, B, threads, io=True):
+    """The oracle as the CPU baseline: compute, + materialise the host CSR (already the
+    oracle's output form), + write the same three files to the same kind of sink."""
+    import oracle
+    t0 = time.perf_counter()
+    Cptr, Cidx, Cval = oracle.spgemm(A, B, threads=threads)
+    t_c = time.perf_counter() - t0
+    res = {"compute_s": round(t_c, 4)}
+    if io:
+        d = tempfile.mkdtemp(prefix="dbd_cpu_io_", dir="/tmp")
+        t0 = time.perf_counter()
+        (Cptr.astype(np.int32) if Cptr[-1] < 2 ** 31 else Cptr).tofile(os.path.join(d, "Ptr.output"))
+        Cidx.tofile(os.path.join(d, "Idx.output"))
+        Cval.tofile(os.path.join(d, "Val.output"))
+        res["io_s"] = round(t_c + time.perf_counter() - t0, 4)
+        shutil.rmtree(d, ignore_errors=True)
+    return res, int(Cptr[-1])
+
+
+def run_web(args):
+    import torch
+
+    import dbd_amd
+    torch.cuda.set_device(0)
+    modes = [m for m in args.modes.split(",") if m]
+    rows = []
+    for s in (2.1, 1.66):
+        A = dbd_amd.gen_web(281903, zipf_s=s, seed=1)
+        log(f"web s={s}: nnz {A.nnz}")
+        P, info, win = _windows(A, None, 4096, np.float64, args.steps, args.warmup, modes)
+        row = {"zipf_s": s, "nnz_A": A.nnz, "products": P, "nnz_C": info["nnz"], "windows": win}
+        if not args.no_cpu:
+            th = _threads()
+            cpu, _ = _cpu_windows(A, A, th, io="io" in modes)
+            cpu1, _ = _cpu_windows(A, A, 1, io=False)
+            row["cpu"] = {"threads": th, **cpu, "gflops": round(2 * P / cpu["compute_s"] / 1e9, 3),
+                          "compute_s_1thread": cpu1["compute_s"]}
+        rows.append(row)
+        log(json.dumps(row))
+    print(json.dumps({"metric": "SpGEMM GFLOPS (2*products/s) per window", "workload":
+                      "web-like 281,903^2, C=A*A, FP64, 4096-row blocks", "unit": "GFLOPS", "runs": rows}))
+
+
+def run_rmat22(args):
+    import torch
+
+    import dbd_amd
+    import oracle
+    torch.cuda.set_device(0)
+    sc = args.rmat_scale
+    t0 = time.perf_counter()
+    A = dbd_amd.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=1)
+    log(f"rmat{sc}: n={A.shape[0]} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
+    ctx = dbd_amd.Context(block_rows=512, dtype=np.float64, device_gib=64, host_gib=256, retain=False)
+    ctx.set_operands(A)
+    info = ctx.plan()
+    P = info["products"]
+    log(f"products {P}, categories {info['sym_rows']}")
+    t, tm = _timed(ctx, "compute", max(1, args.steps), min(1, args.warmup))
+    info = ctx.plan_info()
+    res = {"metric": "SpGEMM GFLOPS (2*products/s), kernel window", "workload":
+           f"R-MAT scale {sc}, edge factor 16, (.57,.19,.19,.05), C=A*A, FP64, 512-row blocks, device budget 64 GB",
+           "n": A.shape[0], "nnz_A": A.nnz, "products": P, "nnz_C": info["nnz"],
+           "kernel": {"ms": round(t * 1e3, 2), "gflops": round(2 * P / t / 1e9, 2), "chunks": tm["chunks"],
+                      "num_rows_per_category": info["num_rows"]}}
+    log(json.dumps(res["kernel"]))
+    ctx.close()
+    # D2H and file windows on a bounded leading row range (C of the full product is ~1 TB)
+    modes = [m for m in args.modes.split(",") if m]
+    if "d2h" in modes or "io" in modes:
+        r = min(A.shape[0], 1 << 18)
+        As = dbd_amd.CSR(A.ptr[:r + 1].copy(), A.idx[:A.ptr[r]].copy(), A.val[:A.ptr[r]].copy(), (r, A.shape[1]))
+        Ps, infs, win = _windows(As, A, 512, np.float64, 1, 1, modes, device_gib=64, host_gib=128)
+        res["leading_rows"] = {"rows": r, "products": Ps, "nnz_C": infs["nnz"], "windows": win}
+        log(json.dumps(res["leading_rows"]))
+        if not args.no_cpu:
+            th = _threads()
+            cpu, _ = _cpu_windows(As, A, th, io="io" in modes)
+            res["leading_rows"]["cpu"] = {"threads": th, **cpu, "gflops": round(2 * Ps / cpu["compute_s"] / 1e9, 3)}
+    if not args.no_cpu:
+        # CPU rate on a bounded sample of rows of the whole product
+        Prow = oracle.row_products(A.ptr, A.idx, A.ptr)
+        rng = np.random.default_rng(0)
+        sel = np.sort(rng.choice(A.shape[0], 20000, replace=False)).astype(np.int32)
+        t0 = time.perf_counter()
+        oracle.spgemm_rows(sel, A, A, threads=_threads())
+        dt = time.perf_counter() - t0
+        res["cpu_sample"] = {"rows": int(sel.size), "products": int(Prow[sel].sum()), "s": round(dt, 3),
+                             "gflops": round(2 * Prow[sel].sum() / dt / 1e9, 3), "threads": _threads()}
+    print(json.dumps(res))
+
+
+def _make_batch(n_mats, seed=2024, lo=1000, hi=200000):
+    import dbd_amd
+    rng = np.random.default_rng(seed)
+    As, Bs = [], []
+    for i in range(n_mats):
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        m = float(rng.uniform(4, 64))
+        As.append(dbd_amd.gen_skewed(n, m, 1.8, seed=seed * 10007 + 2 * i, dtype=np.float32))
+        Bs.append(dbd_amd.gen_skewed(n, m, 1.8, seed=seed * 10007 + 2 * i + 1, dtype=np.float32))
+    return As, Bs
+
+
+def run_batch(args):
+    import torch
+
+    import oracle
+    from dbd_amd import batch
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    As, Bs = _make_batch(args.batch_members)
+    A_cat, B_cat, lay = batch.concat(As, Bs)
+    P = int(oracle.row_products(A_cat.ptr, A_cat.idx, B_cat.ptr).sum())
+    log(f"batch of {len(As)}: rows {A_cat.shape[0]}, nnz A {A_cat.nnz}, products {P}, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    free, total = torch.cuda.mem_get_info()
+    need = P * 8 + (A_cat.nnz + B_cat.nnz) * 16 + A_cat.shape[0] * 80
+    if need > 0.9 * free:
+        raise SystemExit(f"batch needs ~{need / 2**30:.0f} GiB of device memory, {free / 2**30:.0f} GiB free")
+    runner = batch.BatchRunner(A_cat, B_cat, block_rows=4096, tune=True)
+    runner.launch()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        runner.launch()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    nnzC = int(runner.Cptr[-1].item())
+    runner.close()
+    del runner
+    torch.cuda.empty_cache()
+    ctx, run = batch.plain_each(As, Bs, block_rows=4096)
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    ctx.close()
+    res = {"metric": "batched SpGEMM GFLOPS (2*products/s)", "workload":
+           f"{len(As)} independent C_i = A_i*B_i, FP32, rows log-U[1e3,2e5], mean nnz/row U[4,64], Zipf(1.8)",
+           "members": len(As), "rows": A_cat.shape[0], "products": P, "nnz_C": nnzC,
+           "batched_graph": {"ms": round(t_batch * 1e3, 2), "gflops": round(2 * P / t_batch / 1e9, 2)},
+           "plain_per_matrix": {"ms": round(t_plain * 1e3, 2), "gflops": round(2 * P / t_plain / 1e9, 2)},
+           "speedup": round(t_plain / t_batch, 3)}
+    print(json.dumps(res))
+
+
+def run_fwdbwd(args):
+    import torch
+
+    import dbd_amd
+    from dbd_amd.autograd import spgemm_values
+    torch.cuda.set_device(0)
+    A = dbd_amd.gen_rmat(18, 16, seed=1, add_identity=True, row_normalize=True, dtype=np.float32)
+    B = dbd_amd.gen_rmat(18, 16, seed=2, dtype=np.float32)
+    Ad, Bd = dbd_amd.DeviceCSR.from_host(A), dbd_amd.DeviceCSR.from_host(B)
+    av = Ad.val.clone().requires_grad_(True)
+    bv = Bd.val.clone().requires_grad_(True)
+
+    def fwd():
+        return spgemm_values(Ad.ptr, Ad.idx, av, A.shape, Bd.ptr, Bd.idx, bv, B.shape, block_rows=512)
+
+    Cv, Cp, Ci = fwd()
+    W = torch.randn(Cv.shape, dtype=torch.float32, device="cuda")
+    P = int(np.diff(B.ptr)[A.idx].sum())
+    steps = max(1, args.steps)
+    for _ in range(args.warmup):
+        with torch.no_grad():
+            fwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        with torch.no_grad():
+            fwd()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / steps
+    for _ in range(args.warmup):
+        av.grad = bv.grad = None
+        c, _, _ = fwd()
+        (c * W).sum().backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        av.grad = bv.grad = None
+        c, _, _ = fwd()
+        (c * W).sum().backward()
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / steps
+    res = {"metric": "forward+backward / forward time", "workload":
+           "R-MAT scale 18 (row-normalised + self loops) x independent R-MAT scale 18, FP32, autograd",
+           "nnz_A": A.nnz, "nnz_B": B.nnz, "products": P, "nnz_C": int(Ci.numel()),
+           "forward_ms": round(t_f * 1e3, 2), "fwd_bwd_ms": round(t_fb * 1e3, 2),
+           "ratio": round(t_fb / t_f, 3), "forward_gflops": round(2 * P / t_f / 1e9, 2)}
+    print(json.dumps(res))

@@ -87,6 +87,8 @@ SIGNATURES = {
     "dbd_batch_prepare": (I32, [P, CSRP, CSRP, C.POINTER(I64)]),
     "dbd_batch_launch": (I32, [P, P, P, P]),
     "dbd_autotune": (I32, [P, C.POINTER(I32)]),
+    "dbd_table_count": (I32, []),
+    "dbd_set_table": (I32, [P, I32]),
     "dbd_plan_blocks": (I32, [P, I64, I32, P, P, C.POINTER(dbd_plan_info)]),
     "dbd_analyze_block": (I32, [I64, I64]),
     "dbd_read_matrix": (I32, [C.c_char_p, I32, I32, CSRP]),
@@ -103,6 +105,14 @@ SIGNATURES = {
 def _load() -> C.CDLL:
     if not os.path.exists(lib_path):
         raise ImportError(f"{lib_path} is not built; run `python -m dbd_amd.build` (hipcc, gfx950)")
+    # One HIP runtime per process: torch ships its own libamdhip64 and asks for it by the
+    # unversioned name, so if libdbd loaded the system copy first, torch would load a second
+    # runtime and device pointers would not be shared. Loading torch first makes libdbd's
+    # libamdhip64.so.7 resolve to torch's copy (same soname).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lb = C.CDLL(lib_path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lb, name)

@@ -316,6 +316,13 @@ class Context:
         check(lib.dbd_backward(self._h, Cptr.data_ptr(), Cidx.data_ptr(), dC.data_ptr(),
                                dA.data_ptr() if dA is not None else None, dB.data_ptr() if dB is not None else None))
 
+    @staticmethod
+    def table_count() -> int:
+        return int(lib.dbd_table_count())
+
+    def set_table(self, table_id: int) -> None:
+        check(lib.dbd_set_table(self._h, table_id))
+
     def autotune(self) -> int:
         ch = C.c_int32()
         check(lib.dbd_autotune(self._h, C.byref(ch)))

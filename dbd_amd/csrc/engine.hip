@@ -515,6 +515,11 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
   if (!c->ops) return set_error(DBD_ERR_STATE, "dbd_run: operands not set");
   if (mode < 0 || mode > 2) return set_error(DBD_ERR_INVALID, "dbd_run: mode must be 0, 1 or 2");
   CHK(ensure_events(c));
+  if (c->o.tune && !c->tuned_set) {
+    int32_t id = 0;
+    CHK(dbd_autotune(c, &id));
+    c->tuned_set = true;
+  }
   dbd_timing t{};
   t.h2d_s = c->h2d_s;
   c->dev_result = c->host_result = false;
@@ -853,6 +858,7 @@ dbd_status dbd_set_operands(dbd_ctx* c, const dbd_csr* A, const dbd_csr* B) {
                      (long long)A->nrows, (long long)A->ncols, (long long)B->nrows, (long long)B->ncols);
   c->ops = false;
   c->planned = c->symb = false;
+  c->tuned_set = false;
   c->dev_result = c->host_result = false;
   const double t0 = now_s();
   CHK(upload(c, A, c->A, c->upA, c->a_bytes));
@@ -1064,6 +1070,19 @@ dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* 
 }  // extern "C"
 
 extern "C" {
+int32_t dbd_table_count(void) { return (int32_t)candidate_tables().size(); }
+
+dbd_status dbd_set_table(dbd_ctx* c, int32_t id) {
+  clear_error();
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  if (id < 0 || id >= (int32_t)candidate_tables().size())
+    return set_error(DBD_ERR_INVALID, "table id %d out of range [0, %d)", id, (int)candidate_tables().size());
+  c->tb = candidate_tables()[(size_t)id];
+  c->planned = c->symb = false;
+  c->tuned_set = true;  // an explicit choice is not overridden by opts.tune
+  return DBD_OK;
+}
+
 dbd_status dbd_autotune(dbd_ctx* c, int32_t* chosen) {
   clear_error();
   if (!c || !c->ops) return set_error(DBD_ERR_STATE, "dbd_autotune: operands not set");

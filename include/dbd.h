@@ -165,6 +165,10 @@ dbd_status dbd_batch_launch(dbd_ctx* ctx, int64_t* d_Cptr, int32_t* d_Cidx, void
 /* Autotuner: time candidate category tables on the operands set in ctx and keep the fastest
  * for this problem class (cached in-process). Returns the chosen table's id. */
 dbd_status dbd_autotune(dbd_ctx* ctx, int32_t* chosen);
+/* Candidate category tables (thresholds, hash-table sizes, lanes per B row) the autotuner
+ * chooses from; dbd_set_table selects one explicitly (results do not depend on the choice). */
+int32_t dbd_table_count(void);
+dbd_status dbd_set_table(dbd_ctx* ctx, int32_t table_id);
 
 /* ---- host-only logic (no device needed) ------------------------------------------ */
 /* Blocks from per-row products (divide_Blocks, Z/main.cpp:259-296) and reference criteria

@@ -187,12 +187,20 @@ def test_er_16_per_row_sampled_full_size(gpu_ok):
 
 
 def test_autotuned_tables_give_identical_results(gpu_ok):
-    """Whatever table the autotuner picks, the result is the same (structure exact)."""
+    """Property: every candidate category table (the autotuner's search space) gives the same
+    structure and oracle-exact values; the tuned choice is one of them."""
     A = csr_sp(_skewed_matrix(4000, 4000, {1: 1500, 2: 3000}, 41))
     ref = reference(A)
+    for tid in range(dbd_amd.Context.table_count()):
+        with dbd_amd.Context(block_rows=512) as ctx:
+            ctx.set_table(tid)
+            ctx.set_operands(A)
+            ctx.run("compute")
+            compare(ctx.result(), ref, np.float64, f"table {tid}")
     with dbd_amd.Context(block_rows=512, tune=True) as ctx:
         ctx.set_operands(A)
         chosen = ctx.autotune()
         ctx.run("compute")
         C = ctx.result()
+    assert 0 <= chosen < dbd_amd.Context.table_count()
     compare(C, ref, np.float64, f"tuned table {chosen}")
