@@ -252,6 +252,12 @@ struct Writer {
     for (auto& t : th) t.join();
     th.clear();
   }
+  // Early returns (a failed HIP call mid-run) must not leave joinable threads or open files.
+  ~Writer() {
+    if (!th.empty()) finish();
+    if (fd_idx >= 0) close(fd_idx);
+    if (fd_val >= 0) close(fd_val);
+  }
 };
 
 }  // namespace
@@ -599,17 +605,11 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
       std::string pi = d + "Idx.output", pv = d + "Val.output";
       wr.fd_idx = open(pi.c_str(), O_CREAT | O_WRONLY | O_TRUNC, 0644);
       wr.fd_val = open(pv.c_str(), O_CREAT | O_WRONLY | O_TRUNC, 0644);
-      if (wr.fd_idx < 0 || wr.fd_val < 0) {
-        if (wr.fd_idx >= 0) close(wr.fd_idx);
-        if (wr.fd_val >= 0) close(wr.fd_val);
+      if (wr.fd_idx < 0 || wr.fd_val < 0)  // the writer's destructor closes what did open
         return set_error(DBD_ERR_IO, "cannot open %s / %s: %s", pi.c_str(), pv.c_str(), strerror(errno));
-      }
       wr.vsz = c->vsz;
-      if (ftruncate(wr.fd_idx, (off_t)(4 * nnzC)) != 0 || ftruncate(wr.fd_val, (off_t)(c->vsz * nnzC)) != 0) {
-        close(wr.fd_idx);
-        close(wr.fd_val);
+      if (ftruncate(wr.fd_idx, (off_t)(4 * nnzC)) != 0 || ftruncate(wr.fd_val, (off_t)(c->vsz * nnzC)) != 0)
         return set_error(DBD_ERR_IO, "cannot size output files: %s", strerror(errno));
-      }
       wr.start(c->o.writer_threads > 0 ? c->o.writer_threads : 4, nslots);
     }
   }
@@ -656,8 +656,9 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
   HIPCHK(hipStreamSynchronize(s));
   if (mode == DBD_MODE_IO) {
     wr.finish();
-    close(wr.fd_idx);
-    close(wr.fd_val);
+    const bool closed_ok = (close(wr.fd_idx) == 0) & (close(wr.fd_val) == 0);
+    wr.fd_idx = wr.fd_val = -1;
+    if (!closed_ok) return set_error(DBD_ERR_IO, "closing the output files failed: %s", strerror(errno));
     if (wr.failed) return set_error(DBD_ERR_IO, "Error during File Write: %s", wr.why.c_str());
     // Ptr.output last, like write_Ptr (Z/writeFile.cpp:177-197)
     const double w0 = now_s();

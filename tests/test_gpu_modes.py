@@ -129,7 +129,7 @@ def test_cli_a_times_b_fp64(gpu_ok, tmp_path):
             f.write(f"{M.shape[0] - 1} {M.shape[1] - 1} {M.nnz}\n")
             rows = np.repeat(np.arange(M.shape[0]), np.diff(M.ptr))
             for r_, c_, v_ in zip(rows, M.idx, M.val):
-                f.write(f"{r_} {c_} {v_!r}\n")
+                f.write(f"{r_} {c_} {float(v_):.17g}\n")
     binp = os.path.join(ROOT, "dbd_amd", "bin", "SSpMM")
     r = subprocess.run([binp, "--fp64", "--no-cache", "--block-rows", "512", str(tmp_path / "a.mtx"),
                         str(tmp_path / "b.mtx")], capture_output=True, text=True, cwd=tmp_path, timeout=120)
