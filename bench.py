@@ -35,15 +35,16 @@ def log(msg):
 
 
 def numeric_bytes(m, nnzA, P, nnzC, vsz=8):
-    """Algorithmic HBM bytes of one numeric pass (layout: int64 row pointers, int32 columns):
-    A.ptr pair 8(m+1), row list 4m, A idx+val (4+v)nnzA, B.ptr pair 16 nnzA, B idx+val
-    gathers (4+v)P, C.ptr pair 8(m+1), C idx+val writes (4+v)nnzC."""
-    return 8 * (m + 1) + 4 * m + (4 + vsz) * nnzA + 16 * nnzA + (4 + vsz) * P + 8 * (m + 1) + (4 + vsz) * nnzC
+    """Algorithmic HBM bytes of one numeric pass (int64 row pointers, int32 columns, v-byte
+    values), independent of how the kernels lay out their workspace: A.ptr 8(m+1), A idx+val
+    (4+v)nnzA, B.ptr pair 8 nnzA, B idx+val gathers (4+v)P, C.ptr 8(m+1), C idx+val writes
+    (4+v)nnzC. Workspace the kernels add (row lists, per-nonzero B descriptors) is not counted."""
+    return 8 * (m + 1) + (4 + vsz) * nnzA + 8 * nnzA + (4 + vsz) * P + 8 * (m + 1) + (4 + vsz) * nnzC
 
 
 def symbolic_bytes(m, nnzA, P):
-    """A.ptr 8(m+1), row list 4m, A idx 4 nnzA, B.ptr pair 16 nnzA, B idx gathers 4P, nnz write 8m."""
-    return 8 * (m + 1) + 4 * m + 4 * nnzA + 16 * nnzA + 4 * P + 8 * m
+    """A.ptr 8(m+1), A idx 4 nnzA, B.ptr pair 8 nnzA, B idx gathers 4P, C row counts 8(m+1)."""
+    return 8 * (m + 1) + 4 * nnzA + 8 * nnzA + 4 * P + 8 * (m + 1)
 
 
 def cpu_baseline(A, B, target_s=12.0):

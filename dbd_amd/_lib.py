@@ -5,7 +5,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-lib_path = os.path.join(HERE, "libdbd.so")
+# DBD_LIBRARY selects another build of the same library (an A/B variant built with
+# `python -m dbd_amd.build --lib-out PATH -D NAME=VALUE`); default: the in-tree build.
+lib_path = os.environ.get("DBD_LIBRARY") or os.path.join(HERE, "libdbd.so")
 
 MODE_COMPUTE, MODE_D2H, MODE_IO = 0, 1, 2
 F32, F64 = 0, 1

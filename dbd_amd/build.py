@@ -41,24 +41,32 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{msg}")
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = True, lib_out: str | None = None, defines: list[str] | None = None,
+          cli: bool = True) -> str:
+    """Compile everything; `lib_out` / `defines` build an alternative library (e.g. a kernel
+    tuning variant) without touching the default one."""
+    lib = lib_out or LIB
+    bdir = BUILD if lib_out is None else lib_out + ".build"
+    os.makedirs(bdir, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
+    dflags = [f"-D{d}" for d in (defines or [])]
     objs = {
-        "kernels.o": [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip")],
-        "engine.o": [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip")],
+        "kernels.o": [HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, "kernels.hip")],
+        "engine.o": [HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, "engine.hip")],
         "host.o": [CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "host.cpp")],
     }
     jobs = []
     with ThreadPoolExecutor(max_workers=3) as ex:
         for name, cmd in objs.items():
-            jobs.append(ex.submit(_run, cmd + ["-o", os.path.join(BUILD, name)]))
+            jobs.append(ex.submit(_run, cmd + ["-o", os.path.join(bdir, name)]))
         for j in jobs:
             j.result()
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp,
-          *[os.path.join(BUILD, n) for n in objs], "-lpthread"])
-    os.replace(tmp, LIB)
+          *[os.path.join(bdir, n) for n in objs], "-lpthread"])
+    os.replace(tmp, lib)
+    if not cli:
+        return lib
     modes = {"compute": ["-DDBD_CLI_MODE=0"], "d2h": ["-DDBD_CLI_MODE=1"],
              "SSpMM": ["-DDBD_CLI_MODE=2"], "debug": ["-DDBD_CLI_MODE=2", "-DDBD_CLI_DEBUG"]}
     with ThreadPoolExecutor(max_workers=4) as ex:
@@ -74,8 +82,13 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib-out", default=None, help="write the library here instead of dbd_amd/libdbd.so")
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor define")
+    a = ap.parse_args()
     try:
-        build()
+        build(lib_out=a.lib_out, defines=a.defines, cli=a.lib_out is None)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

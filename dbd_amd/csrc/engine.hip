@@ -283,7 +283,7 @@ struct dbd_ctx {
   bool tuned_set = false;
   bool planned = false, symb = false;
   int64_t nr = 0, nb = 0, nwg = 0;
-  DevBuf P, Pptr, nnz, Cptr, rows_sym, rows_num, cnt, offs_sym, offs_num, tmp, cursor, blk, err;
+  DevBuf P, Pptr, nnz, Cptr, rows_sym, rows_num, cnt, offs_sym, offs_num, tmp, cursor, blk, err, bdesc_s, bdesc_l;
   std::vector<int64_t> h_blk, h_blockP, h_blockC, h_offs_num, h_sym_tot;
   dbd_plan_info info{};
 
@@ -348,9 +348,15 @@ dbd_status plan(dbd_ctx* c) {
   CHK(c->tmp.ensure(8 * (size_t)scan_tmp_elems((int64_t)ncnt + nr + 1)));
   CHK(c->blk.ensure(8 * (size_t)(std::max(c->nb, (int64_t)kMaxCats) + 2)));
   CHK(c->err.ensure(sizeof(int)));
+  CHK(c->bdesc_s.ensure(8 * (size_t)(c->A.nnz + 1)));
+  CHK(c->bdesc_l.ensure(4 * (size_t)(c->A.nnz + 1)));
+  c->A.bstart = c->bdesc_s.as<int64_t>();
+  c->A.blen = c->bdesc_l.as<int32_t>();
+  if (c->same) c->B = c->A;
   launch_fill_i32(c->err.as<int32_t>(), 1, 0, s);
 
-  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), s);
+  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), (int64_t*)c->A.bstart,
+                      (int32_t*)c->A.blen, s);
   launch_exclusive_scan(c->P.as<int64_t>(), c->Pptr.as<int64_t>(), nr, c->tmp.as<int64_t>(), s);
   launch_gather_strided(c->Pptr.as<int64_t>(), nr, BR, c->nb, c->blk.as<int64_t>(), s);
   c->h_blk.assign((size_t)c->nb + 1, 0);
@@ -513,6 +519,7 @@ int64_t fixed_device_bytes(dbd_ctx* c) {
   int64_t s = c->a_bytes + (c->same ? 0 : c->b_bytes);
   s += (int64_t)c->P.bytes + c->Pptr.bytes + c->nnz.bytes + c->Cptr.bytes + c->rows_sym.bytes + c->rows_num.bytes;
   s += (int64_t)c->cnt.bytes + c->offs_sym.bytes + c->offs_num.bytes + c->tmp.bytes + c->cursor.bytes;
+  s += (int64_t)c->bdesc_s.bytes + c->bdesc_l.bytes;
   return s;
 }
 
@@ -807,7 +814,7 @@ dbd_status dbd_destroy(dbd_ctx* c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   if (c->cst) (void)hipStreamSynchronize(c->cst);
   for (auto* b : {&c->P, &c->Pptr, &c->nnz, &c->Cptr, &c->rows_sym, &c->rows_num, &c->cnt, &c->offs_sym, &c->offs_num,
-                  &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng})
+                  &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng, &c->bdesc_s, &c->bdesc_l})
     b->release();
   for (int i = 0; i < 3; ++i) {
     c->upA[i].release();
@@ -1003,7 +1010,8 @@ static int persistent_grid(dbd_ctx* c, const KernelShape& k) {
 static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval) {
   const int64_t nr = c->nr;
   launch_fill_i32(c->err.as<int32_t>(), 1, 0, s);
-  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), s);
+  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), (int64_t*)c->A.bstart,
+                      (int32_t*)c->A.blen, s);
   const CatTable& ct = c->tb.sym_ct;
   launch_bin_count(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->cnt.as<int64_t>(), s);
   launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_sym.as<int64_t>(), (int64_t)ct.n * c->nwg, c->tmp.as<int64_t>(), s);

@@ -24,6 +24,10 @@ struct Mat {
   const int32_t* idx;
   const void* val;
   int64_t nrows, ncols, nnz;
+  // For A only: per-nonzero descriptor of the B row it selects (start, length), written by the
+  // row analysis in A's order so later passes read it sequentially instead of gathering B.ptr.
+  const int64_t* bstart = nullptr;
+  const int32_t* blen = nullptr;
 };
 
 // One category's kernel shape, chosen by the planner/autotuner.
@@ -44,8 +48,10 @@ void launch_exclusive_scan(const int64_t* in, int64_t* out, int64_t n, int64_t* 
 void launch_gather_strided(const int64_t* ptr, int64_t n, int64_t stride, int64_t nb, int64_t* out, hipStream_t s);
 
 // ---- planning ----
-// P[i] = sum_{k in A_i} len(B_k); nnz[i] = 0 where P[i] == 0.
-void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, hipStream_t s);
+// P[i] = sum_{k in A_i} len(B_k); nnz[i] = 0 where P[i] == 0; per A nonzero j with column k:
+// bstart[j] = B.ptr[k], blen[j] = len(B_k) (both nullable).
+void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, int64_t* bstart, int32_t* blen,
+                         hipStream_t s);
 // Stable counting sort of rows [0, n) by category of min(key[i], cap).
 // counts/offs have ncat * nwg (+1) int64 where nwg = bin_tiles(n).
 int64_t bin_tiles(int64_t n);

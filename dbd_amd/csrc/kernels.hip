@@ -107,6 +107,34 @@ struct OpAdd {
   __device__ T operator()(T a, T b) const { return a + b; }
 };
 
+// Whole-wave reduction of an int without LDS: DPP within rows of 16 lanes (quad swaps, half
+// mirror, mirror), then the four row results through readlane. All 64 lanes must be active.
+template <typename Op>
+__device__ __forceinline__ int wave_reduce_dpp(int x, Op op) {
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  x = op(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false));  // row_mirror
+  const int r0 = __builtin_amdgcn_readlane(x, 0), r1 = __builtin_amdgcn_readlane(x, 16);
+  const int r2 = __builtin_amdgcn_readlane(x, 32), r3 = __builtin_amdgcn_readlane(x, 48);
+  return op(op(r0, r1), op(r2, r3));
+}
+
+// Exclusive prefix over the wave of a small non-negative int (< 2^NBITS), bit by bit with
+// ballots and mbcnt; no LDS traffic. `total` receives the wave sum.
+template <int NBITS>
+__device__ __forceinline__ int wave_exscan_small(int x, int& total) {
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int b = 0; b < NBITS; ++b) {
+    const uint64_t m = __ballot((x >> b) & 1);
+    pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+    tot += __popcll(m) << b;
+  }
+  total = tot;
+  return pre;
+}
+
 // Mask of the G lanes (within this wave) that share this lane's subgroup.
 template <int G>
 __device__ __forceinline__ uint64_t subgroup_mask() {
@@ -132,13 +160,10 @@ __device__ __forceinline__ int hash_insert_key(int* hk, int c) {
   uint32_t h = hslot<T>(c);
 #pragma unroll 1
   for (int probe = 0; probe < T; ++probe) {
-    int cur = __hip_atomic_load(hk + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == c) return 0;
-    if (cur == kEmpty) {
-      int old = atomicCAS(hk + h, kEmpty, c);
-      if (old == kEmpty) return 1;
-      if (old == c) return 0;
-    }
+    // compare-and-swap first: at load <= 1/2 most probes find the slot empty or holding c
+    const int old = atomicCAS(hk + h, kEmpty, c);
+    if (old == kEmpty) return 1;
+    if (old == c) return 0;
     h = (h + 1) & (T - 1);
   }
   return -1;
@@ -155,12 +180,8 @@ __device__ __forceinline__ bool hash_insert_add(int* hk, V* hv, int c, V v) {
   uint32_t h = hslot<T>(c);
 #pragma unroll 1
   for (int probe = 0; probe < T; ++probe) {
-    int cur = __hip_atomic_load(hk + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == kEmpty) {
-      int old = atomicCAS(hk + h, kEmpty, c);
-      cur = (old == kEmpty) ? c : old;
-    }
-    if (cur == c) {
+    const int old = atomicCAS(hk + h, kEmpty, c);
+    if (old == kEmpty || old == c) {
       lds_add(hv + h, v);
       return true;
     }
@@ -168,6 +189,70 @@ __device__ __forceinline__ bool hash_insert_add(int* hk, V* hv, int c, V v) {
   }
   return false;
 }
+
+// Product sink of the hash kernels: tracks the column range, inserts keys (and accumulates
+// values when VALS). With BATCH, batch() issues the first probe of all U products before
+// looking at any result (U LDS compare-and-swaps in flight together); collisions then continue
+// probing. Without it the products are inserted one after another.
+template <int T, typename V, bool VALS, bool BATCH = true>
+struct HashSink {
+  int* K;
+  V* H;
+  int cmin = INT_MAX, cmax = -1, created = 0, bad = 0;
+  __device__ __forceinline__ void probe_from(uint32_t h, int c, V v) {
+#pragma unroll 1
+    for (int probe = 1; probe < T; ++probe) {
+      h = (h + 1) & (T - 1);
+      const int old = atomicCAS(K + h, kEmpty, c);
+      if (old == kEmpty || old == c) {
+        created += (old == kEmpty);
+        if (VALS) lds_add(H + h, v);
+        return;
+      }
+    }
+    bad = 1;
+  }
+  template <int U>
+  __device__ __forceinline__ void batch(const int (&c)[U], const V (&v)[U]) {
+    if constexpr (!BATCH) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (c[u] >= 0) one(c[u], v[u]);
+      return;
+    }
+    uint32_t h[U];
+    int old[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      h[u] = hslot<T>(c[u]);
+      old[u] = (c[u] >= 0) ? atomicCAS(K + h[u], kEmpty, c[u]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c[u] < 0) continue;
+      cmin = min(cmin, c[u]);
+      cmax = max(cmax, c[u]);
+      if (old[u] == kEmpty || old[u] == c[u]) {
+        created += (old[u] == kEmpty);
+        if (VALS) lds_add(H + h[u], v[u]);
+      } else {
+        probe_from(h[u], c[u], v[u]);
+      }
+    }
+  }
+  __device__ __forceinline__ void one(int c, V v) {
+    cmin = min(cmin, c);
+    cmax = max(cmax, c);
+    const uint32_t h = hslot<T>(c);
+    const int old = atomicCAS(K + h, kEmpty, c);
+    if (old == kEmpty || old == c) {
+      created += (old == kEmpty);
+      if (VALS) lds_add(H + h, v);
+    } else {
+      probe_from(h, c, v);
+    }
+  }
+};
 
 // Lookup of a key known to be present; returns slot or -1.
 template <int T>
@@ -183,62 +268,206 @@ __device__ __forceinline__ int hash_find(const int* hk, int c) {
   return -1;
 }
 
-// Walk every product of one row for a group of NT >= 64 threads. Each wave of the group loads
-// up to 64 A entries at once (lane j: column k, value, B row start and length), then its
-// 64/G subgroups take entries through wave shuffles, U entries per step, issuing all U B-row
-// gathers before handing the products to f(col, val). B rows longer than G lanes continue in
-// a tail loop. Control flow around the shuffles is wave-uniform.
-//   f(p, c, v, va): p = position of the product's B entry, c = its column, v = A_ik * B_kj,
-//   va = A_ik. WITHV = false skips the value loads (symbolic pass).
-template <int NT, int G, int U, bool WITHV, typename V, typename F>
-__device__ __forceinline__ void for_row_products(int t, int64_t a0, int64_t a1, const int32_t* __restrict__ Aidx,
-                                                 const V* __restrict__ Aval, const int64_t* __restrict__ Bptr,
-                                                 const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F&& f) {
-  static_assert(NT >= 64 && 64 % G == 0, "wave-level walk");
+// Descriptor of the A entry a lane holds: start and length of the selected B row, A's value.
+template <typename V>
+struct ADesc {
+  int64_t b0;
+  int len;
+  V va;
+};
+
+// Lane `lane` loads the descriptor of A entry base + lane (empty past a1). These loads are
+// sequential in A's order (coalesced).
+template <bool WITHV, typename V>
+__device__ __forceinline__ ADesc<V> load_desc(int64_t base, int64_t a1, int lane, const int64_t* __restrict__ bstart,
+                                              const int32_t* __restrict__ blen, const V* __restrict__ Aval) {
+  const int64_t j = base + lane;
+  const bool ok = j < a1;
+  ADesc<V> d;
+  d.b0 = ok ? bstart[j] : 0;
+  d.len = ok ? blen[j] : 0;
+  d.va = (WITHV && ok) ? Aval[j] : V(0);
+  return d;
+}
+
+// One gather step over a chunk of A entries whose descriptors are spread over the wave's
+// lanes: the wave's 64/G subgroups take entries through wave shuffles, U entries per step, and
+// issue the U B-row gathers (the first G entries of each B row). Only the gathered columns and
+// raw B values are kept; the products are formed when the step is consumed.
+template <int U, typename V>
+struct Step {
+  int c[U];
+  V bv[U];
+};
+
+template <int U, typename V>
+struct EntryRef {  // entry u of a step, as read back from the descriptors
+  int64_t pb[U];
+  int pl[U];
+  V pv[U];
+};
+
+template <int G, int U, bool WITHV, typename V>
+__device__ __forceinline__ void step_entries(EntryRef<U, V>& r, int lane, int e0, int cnt, const ADesc<V>& d) {
+  static_assert(64 % G == 0, "subgroups tile the wave");
   constexpr int NG = 64 / G;
-  const int lane = t & 63, wv = t >> 6;
-  const int sg = lane / G, lg = lane % G;
-  for (int64_t base = a0 + 64 * wv; base < a1; base += NT) {
-    const int64_t j = base + lane;
-    const bool ok = j < a1;
-    const int k = ok ? Aidx[j] : 0;
-    const V va = (WITHV && ok) ? Aval[j] : V(0);
-    const int64_t b0 = ok ? Bptr[k] : 0;
-    const int len = ok ? (int)(Bptr[k + 1] - b0) : 0;
-    const int cnt = (int)min((int64_t)64, a1 - base);
-    for (int e0 = 0; e0 < cnt; e0 += NG * U) {
-      int64_t pb[U];
-      int pl[U];
-      V pv[U];
+  const int sg = lane / G;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * NG + sg;
-        const int src = e < 64 ? e : 63;
-        pb[u] = __shfl(b0, src);
-        pl[u] = __shfl(len, src);
-        pv[u] = WITHV ? __shfl(va, src) : V(0);
-        if (e >= cnt) pl[u] = 0;
-      }
-      int c[U];
-      V v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const bool has = lg < pl[u];
-        c[u] = has ? Bidx[pb[u] + lg] : -1;
-        v[u] = (WITHV && has) ? pv[u] * Bval[pb[u] + lg] : V(0);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (c[u] >= 0) f(pb[u] + lg, c[u], v[u], pv[u]);
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        for (int q = G + lg; q < pl[u]; q += G) {
-          const int64_t p = pb[u] + q;
-          f(p, Bidx[p], WITHV ? pv[u] * Bval[p] : V(0), pv[u]);
-        }
-    }
+  for (int u = 0; u < U; ++u) {
+    const int e = e0 + u * NG + sg;
+    const int src = e < 64 ? e : 63;
+    r.pb[u] = __shfl(d.b0, src);
+    r.pl[u] = e < cnt ? __shfl(d.len, src) : 0;
+    r.pv[u] = WITHV ? __shfl(d.va, src) : V(0);
   }
 }
+
+template <int G, int U, bool WITHV, typename V>
+__device__ __forceinline__ void gather_step(Step<U, V>& st, int lane, int e0, int cnt, const ADesc<V>& d,
+                                            const int32_t* __restrict__ Bidx, const V* __restrict__ Bval) {
+  EntryRef<U, V> r;
+  step_entries<G, U, false, V>(r, lane, e0, cnt, d);
+  const int lg = lane % G;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const bool has = lg < r.pl[u];
+    st.c[u] = has ? Bidx[r.pb[u] + lg] : -1;
+    st.bv[u] = (WITHV && has) ? Bval[r.pb[u] + lg] : V(0);
+  }
+}
+
+// Hand a gathered step to the sink: f.batch(c[U], v[U]) for the first G entries of each B row
+// (c < 0 marks no product), f.one(c, v) for the tails of B rows longer than G; v = A_ik * B_kj.
+template <int G, int U, bool WITHV, typename V, typename F>
+__device__ __forceinline__ void consume_step(const Step<U, V>& st, int lane, int e0, int cnt, const ADesc<V>& d,
+                                             const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F& f) {
+  EntryRef<U, V> r;
+  step_entries<G, U, WITHV, V>(r, lane, e0, cnt, d);
+  const int lg = lane % G;
+  V v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) v[u] = WITHV ? r.pv[u] * st.bv[u] : V(0);
+  f.template batch<U>(st.c, v);
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    for (int q = G + lg; q < r.pl[u]; q += G) {
+      const int64_t p = r.pb[u] + q;
+      f.one(Bidx[p], WITHV ? r.pv[u] * Bval[p] : V(0));
+    }
+}
+
+// All products of `cnt` (<= 64) A entries of one chunk, steps of U entries per subgroup.
+// Control flow around the shuffles is wave-uniform. `from` skips steps already consumed.
+template <int G, int U, bool WITHV, typename V, typename F>
+__device__ __forceinline__ void walk_chunk(int lane, int cnt, const ADesc<V>& d, const int32_t* __restrict__ Bidx,
+                                           const V* __restrict__ Bval, F& f, int from = 0) {
+  constexpr int NG = 64 / G;
+  for (int e0 = from; e0 < cnt; e0 += NG * U) {
+    Step<U, V> st;
+    gather_step<G, U, WITHV, V>(st, lane, e0, cnt, d, Bidx, Bval);
+    consume_step<G, U, WITHV, V>(st, lane, e0, cnt, d, Bidx, Bval, f);
+  }
+}
+
+// All products of one row for a group of NT >= 64 threads: wave w handles A entries
+// a0 + 64w, a0 + 64w + NT, ...; `first` holds the descriptors of the wave's first chunk when
+// the caller prefetched them.
+template <int NT, int G, int U, bool WITHV, typename V, typename F>
+__device__ __forceinline__ void for_row_products_pre(int t, int64_t a0, int64_t a1, const ADesc<V>& first,
+                                                     const int64_t* __restrict__ bstart,
+                                                     const int32_t* __restrict__ blen, const V* __restrict__ Aval,
+                                                     const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+                                                     F& f, const Step<U, V>* step0 = nullptr) {
+  static_assert(NT >= 64, "wave-level walk");
+  constexpr int NG = 64 / G;
+  const int lane = t & 63, wv = t >> 6;
+  int64_t base = a0 + 64 * wv;
+  if (base < a1) {
+    const int cnt = (int)min((int64_t)64, a1 - base);
+    int from = 0;
+    if (step0) {  // the caller already issued the first step's gathers
+      consume_step<G, U, WITHV, V>(*step0, lane, 0, cnt, first, Bidx, Bval, f);
+      from = NG * U;
+    }
+    walk_chunk<G, U, WITHV, V>(lane, cnt, first, Bidx, Bval, f, from);
+  }
+  for (base += NT; base < a1; base += NT) {
+    const ADesc<V> d = load_desc<WITHV, V>(base, a1, lane, bstart, blen, Aval);
+    walk_chunk<G, U, WITHV, V>(lane, (int)min((int64_t)64, a1 - base), d, Bidx, Bval, f);
+  }
+}
+
+template <int NT, int G, int U, bool WITHV, typename V, typename F>
+__device__ __forceinline__ void for_row_products(int t, int64_t a0, int64_t a1, const int64_t* __restrict__ bstart,
+                                                 const int32_t* __restrict__ blen, const V* __restrict__ Aval,
+                                                 const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F& f) {
+  const ADesc<V> first = load_desc<WITHV, V>(a0 + 64 * (t >> 6), a1, t & 63, bstart, blen, Aval);
+  for_row_products_pre<NT, G, U, WITHV, V>(t, a0, a1, first, bstart, blen, Aval, Bidx, Bval, f);
+}
+
+// Rows handled by one wave, software-pipelined: the row id two ahead, the row pointers of the
+// next row, and (after the current row's products are in LDS) the next row's A descriptors
+// are loaded early, so only the B gathers stay on each row's critical path.
+template <bool WITHV, typename V>
+struct RowPipe {
+  int64_t slot, stride, end;
+  int row, nrow;
+  int64_t a0, a1, c0, c1;     // current row: A range and output range
+  int64_t na0, na1, nc0, nc1; // next row
+  ADesc<V> d, nd;
+  bool more;
+
+  __device__ __forceinline__ void start(int64_t s0, int64_t st, int64_t e, const int32_t* __restrict__ rows,
+                                        const int64_t* __restrict__ Aptr, const int64_t* __restrict__ Cptr, int lane,
+                                        const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
+                                        const V* __restrict__ Aval) {
+    slot = s0;
+    stride = st;
+    end = e;
+    row = rows[slot];
+    a0 = Aptr[row];
+    a1 = Aptr[row + 1];
+    if (Cptr) {
+      c0 = Cptr[row];
+      c1 = Cptr[row + 1];
+    }
+    d = load_desc<WITHV, V>(a0, a1, lane, bstart, blen, Aval);
+    nrow = (slot + stride < end) ? rows[slot + stride] : 0;
+  }
+  // top of an iteration: row pointers of the next row, id of the one after
+  __device__ __forceinline__ int prefetch_ptrs(const int32_t* __restrict__ rows, const int64_t* __restrict__ Aptr,
+                                               const int64_t* __restrict__ Cptr) {
+    more = slot + stride < end;
+    int nrow2 = 0;
+    if (more) {
+      na0 = Aptr[nrow];
+      na1 = Aptr[nrow + 1];
+      if (Cptr) {
+        nc0 = Cptr[nrow];
+        nc1 = Cptr[nrow + 1];
+      }
+      if (slot + 2 * stride < end) nrow2 = rows[slot + 2 * stride];
+    }
+    return nrow2;
+  }
+  // after the current row's products: the next row's descriptors
+  __device__ __forceinline__ void prefetch_desc(int lane, const int64_t* __restrict__ bstart,
+                                                const int32_t* __restrict__ blen, const V* __restrict__ Aval) {
+    if (more) nd = load_desc<WITHV, V>(na0, na1, lane, bstart, blen, Aval);
+  }
+  __device__ __forceinline__ bool advance(int nrow2) {
+    if (!more) return false;
+    slot += stride;
+    row = nrow;
+    nrow = nrow2;
+    a0 = na0;
+    a1 = na1;
+    c0 = nc0;
+    c1 = nc1;
+    d = nd;
+    return true;
+  }
+};
 
 // Error bits written to the device error word.
 constexpr int kErrTableFull = 1;
@@ -367,7 +596,8 @@ constexpr int kRpLanes = 16;
 __global__ __launch_bounds__(256) void k_row_products(int64_t nr, const int64_t* __restrict__ Aptr,
                                                       const int32_t* __restrict__ Aidx,
                                                       const int64_t* __restrict__ Bptr, int64_t* __restrict__ P,
-                                                      int64_t* __restrict__ nnz) {
+                                                      int64_t* __restrict__ nnz, int64_t* __restrict__ bstart,
+                                                      int32_t* __restrict__ blen) {
   const int l = threadIdx.x % kRpLanes;
   for (int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kRpLanes; row < nr;
        row += (int64_t)gridDim.x * (256 / kRpLanes)) {
@@ -375,7 +605,13 @@ __global__ __launch_bounds__(256) void k_row_products(int64_t nr, const int64_t*
     int64_t s = 0;
     for (int64_t ka = a0 + l; ka < a1; ka += kRpLanes) {
       const int k = Aidx[ka];
-      s += Bptr[k + 1] - Bptr[k];
+      const int64_t b0 = Bptr[k];
+      const int64_t len = Bptr[k + 1] - b0;
+      s += len;
+      if (bstart) {
+        bstart[ka] = b0;
+        blen[ka] = (int32_t)len;
+      }
     }
 #pragma unroll
     for (int o = kRpLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kRpLanes);
@@ -386,11 +622,13 @@ __global__ __launch_bounds__(256) void k_row_products(int64_t nr, const int64_t*
   }
 }
 
-void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, hipStream_t s) {
+void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, int64_t* bstart, int32_t* blen,
+                         hipStream_t s) {
   if (A.nrows <= 0) return;
   int64_t g = (A.nrows * kRpLanes + 255) / 256;
   if (g > 262144) g = 262144;
-  hipLaunchKernelGGL(k_row_products, dim3((unsigned)g), dim3(256), 0, s, A.nrows, A.ptr, A.idx, B.ptr, P, nnz);
+  hipLaunchKernelGGL(k_row_products, dim3((unsigned)g), dim3(256), 0, s, A.nrows, A.ptr, A.idx, B.ptr, P, nnz, bstart,
+                     blen);
 }
 
 // ------------------------------------------------------------------------------------
@@ -469,31 +707,53 @@ constexpr int wg_of() { return NT <= 64 ? 256 : NT; }
 template <int T, int NT, int G>
 __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
-    const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx, const int64_t* __restrict__ Bptr,
+    const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
     const int32_t* __restrict__ Bidx, int64_t* __restrict__ nnz, int* __restrict__ err) {
   constexpr int WG = wg_of<NT>();
   constexpr int GPB = WG / NT;
   static_assert(NT % G == 0 && T % NT == 0, "shape");
   __shared__ int hk[GPB][T];
   __shared__ int sc[GPB][NT > 64 ? NT / 64 : 1];
-  const int g = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int t = threadIdx.x % NT;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);  // wave-uniform: keep row state scalar
   int* K = hk[g];
   const int64_t beg = *pbeg, end = *pend;
+  if constexpr (NT == 64) {
+    const int64_t s0 = beg + (int64_t)blockIdx.x * GPB + g;
+    if (s0 >= end) return;
+    RowPipe<false, float> rp;
+    rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, nullptr, t, bstart, blen, (const float*)nullptr);
+    for (;;) {
+      const int nrow2 = rp.prefetch_ptrs(rows, Aptr, nullptr);
+#pragma unroll
+      for (int q = 0; q < T / 64; ++q) K[(T / 64) * t + q] = kEmpty;
+      wave_sync();
+      HashSink<T, float, false, false> sk{K, nullptr};
+      for_row_products_pre<64, G, 4, false, float>(t, rp.a0, rp.a1, rp.d, bstart, blen, (const float*)nullptr, Bidx,
+                                                   (const float*)nullptr, sk);
+      rp.prefetch_desc(t, bstart, blen, (const float*)nullptr);
+      const int cnt = wave_reduce_dpp(sk.created, OpAdd());
+      const int bad = wave_reduce_dpp(sk.bad, OpMax());
+      if (t == 0) {
+        nnz[rp.row] = cnt;
+        if (bad) atomicOr(err, kErrTableFull);
+      }
+      wave_sync();
+      if (!rp.advance(nrow2)) break;
+    }
+    return;
+  }
   for (int64_t slot = beg + (int64_t)blockIdx.x * GPB + g; slot < end; slot += (int64_t)gridDim.x * GPB) {
     const int row = rows[slot];
 #pragma unroll
     for (int s = t; s < T; s += NT) K[s] = kEmpty;
     gsync<NT>();
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
-    int cnt = 0, bad = 0;
-    for_row_products<NT, G, 4, false, float>(t, a0, a1, Aidx, (const float*)nullptr, Bptr, Bidx,
-                                             (const float*)nullptr, [&](int64_t, int c, float, float) {
-                                               const int r = hash_insert_key<T>(K, c);
-                                               cnt += (r > 0);
-                                               bad |= (r < 0);
-                                             });
-    cnt = Group<NT>::reduce(cnt, OpAdd(), sc[g]);
-    bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
+    HashSink<T, float, false, false> sk{K, nullptr};
+    for_row_products<NT, G, 4, false, float>(t, a0, a1, bstart, blen, (const float*)nullptr, Bidx,
+                                             (const float*)nullptr, sk);
+    const int cnt = Group<NT>::reduce(sk.created, OpAdd(), sc[g]);
+    const int bad = Group<NT>::reduce(sk.bad, OpMax(), sc[g]);
     if (t == 0) {
       nnz[row] = cnt;
       if (bad) atomicOr(err, kErrTableFull);
@@ -580,126 +840,298 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 // numeric: accumulate in an LDS hash, sort by a monotone bucket map + in-bucket rank,
 // emit coalesced
 // ------------------------------------------------------------------------------------
+// Sort the live entries of a one-wave row table (T slots, at most T/2 live) in place and
+// return their count: compact live slots (a lane's write position counts the live slots before
+// its own, so it never passes an unread slot), bucket them with the monotone map
+// b(c) = floor((c - cmin) * NB / range), scan the bucket counts, and rank each key inside its
+// bucket. Counters and the permutation live in the table's upper halves.
+template <typename V, int T>
+__device__ __forceinline__ int wave_sort_table(int t, int* K, V* H, int cmin, int cmax, int& bad) {
+  constexpr int IPT = T / 64;
+  constexpr int NB = T / 2 - 1;
+  constexpr int PER = (NB + 63) / 64;
+  constexpr int IQ = (T / 2 + 63) / 64;
+  const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
+  const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
+  int kk[IPT];
+  V vv[IPT];
+  int live = 0;
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    kk[q] = K[IPT * t + q];
+    vv[q] = H[IPT * t + q];
+    live += (kk[q] != kEmpty);
+  }
+  int n;
+  int pos = wave_exscan_small<4>(live, n);  // live <= IPT <= 8
+#pragma unroll
+  for (int q = 0; q < IPT; ++q)
+    if (kk[q] != kEmpty) {
+      K[pos] = kk[q];
+      H[pos] = vv[q];
+      ++pos;
+    }
+  wave_sync();
+  int* BC = (int*)(H + T / 2);
+  int* perm = K + T / 2;
+  bad |= (n > T / 2);
+  int ik[IQ], ib[IQ], io[IQ], rk[IQ];
+  V iv[IQ];
+#pragma unroll
+  for (int q = 0; q < IQ; ++q) {
+    const int i = t + 64 * q;
+    ik[q] = (i < n && i < T / 2) ? K[i] : kEmpty;
+    iv[q] = (i < n && i < T / 2) ? H[i] : V(0);
+  }
+  for (int s2 = t; s2 <= NB; s2 += 64) BC[s2] = 0;
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < IQ; ++q) {
+    ib[q] = 0;
+    io[q] = 0;
+    if (ik[q] != kEmpty) {
+      const int b = (int)(((uint64_t)(uint32_t)(ik[q] - cmin) * mul) >> 32);
+      ib[q] = b;
+      io[q] = atomicAdd(&BC[b], 1);
+    }
+  }
+  wave_sync();
+  {
+    int loc[PER];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = t * PER + q;
+      loc[q] = (i < NB) ? BC[i] : 0;
+      sum += loc[q];
+    }
+    int total;
+    int pre = wave_exscan_small<__builtin_ctz(T / 2) + 1>(sum, total);  // sum <= T/2
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = t * PER + q;
+      if (i < NB) BC[i] = pre;
+      pre += loc[q];
+    }
+    if (t == 0) BC[NB] = total;
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < IQ; ++q)
+    if (ik[q] != kEmpty) perm[BC[ib[q]] + io[q]] = ik[q];
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < IQ; ++q) {
+    rk[q] = 0;
+    if (ik[q] != kEmpty) {
+      const int s0 = BC[ib[q]], s1 = BC[ib[q] + 1];
+      int r = s0;
+      for (int m = s0; m < s1; ++m) r += (perm[m] < ik[q]);
+      rk[q] = r;
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < IQ; ++q)
+    if (ik[q] != kEmpty) {
+      K[rk[q]] = ik[q];
+      H[rk[q]] = iv[q];
+    }
+  wave_sync();
+  return n > T / 2 ? T / 2 : n;
+}
+
 template <typename V, int T, int NT, int G>
-__global__ __launch_bounds__(wg_of<NT>()) void k_numeric_hash(
+__device__ __forceinline__ void numeric_hash_body(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
-    const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx, const V* __restrict__ Aval,
-    const int64_t* __restrict__ Bptr, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+    const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
+    const V* __restrict__ Aval, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
     const int64_t* __restrict__ Cptr, int64_t cbase, int32_t* __restrict__ Cidx, V* __restrict__ Cval,
     int* __restrict__ err) {
   constexpr int WG = wg_of<NT>();
   constexpr int GPB = WG / NT;
-  constexpr int NB = T / 2;        // buckets
-  constexpr int IPT = T / NT;      // table slots per thread
-  constexpr int PER = (NB + NT - 1) / NT;
+  constexpr int IPT = T / NT;  // table slots per thread
   static_assert(NT % G == 0 && T % NT == 0, "shape");
+  constexpr bool WAVE = (NT == 64);
+  constexpr int NB = T / 2;  // buckets (workgroup path)
+  constexpr int PER = (NB + NT - 1) / NT;
   __shared__ int hk[GPB][T];
   __shared__ V hv[GPB][T];
-  __shared__ int bc[GPB][NB + 1];
+  __shared__ int bc[GPB][WAVE ? 1 : NB + 1];
   __shared__ int sc[GPB][NT > 64 ? NT / 64 : 1];
-  const int g = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int t = threadIdx.x % NT;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);  // wave-uniform
   int* K = hk[g];
   V* H = hv[g];
-  int* BC = bc[g];
   const int64_t beg = *pbeg, end = *pend;
-  for (int64_t slot = beg + (int64_t)blockIdx.x * GPB + g; slot < end; slot += (int64_t)gridDim.x * GPB) {
-    const int row = rows[slot];
+  if constexpr (WAVE) {
+    // one wave per row, rows software-pipelined (RowPipe); the table is sorted by
+    // wave_sort_table while the next row's descriptors are in flight
+    const int64_t s0 = beg + (int64_t)blockIdx.x * GPB + g;
+    if (s0 >= end) return;
+    RowPipe<true, V> rp;
+    rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, Cptr, t, bstart, blen, Aval);
+    // the first gather step of each row is issued before the previous row's output stores: on
+    // this architecture stores share the load counter, so a wait for gathers issued after the
+    // stores would also wait for the stores
+    Step<4, V> st0;
+    gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx, Bval);
+    for (;;) {
+      const int nrow2 = rp.prefetch_ptrs(rows, Aptr, Cptr);
 #pragma unroll
-    for (int s = t; s < T; s += NT) {
-      K[s] = kEmpty;
-      H[s] = V(0);
-    }
-    for (int s = t; s <= NB; s += NT) BC[s] = 0;
-    gsync<NT>();
-    const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
-    int cmin = INT_MAX, cmax = -1, bad = 0;
-    for_row_products<NT, G, 4, true, V>(t, a0, a1, Aidx, Aval, Bptr, Bidx, Bval, [&](int64_t, int c, V v, V) {
-      cmin = min(cmin, c);
-      cmax = max(cmax, c);
-      bad |= !hash_insert_add<T>(K, H, c, v);
-    });
-    gsync<NT>();
-    cmin = Group<NT>::reduce(cmin, OpMin(), sc[g]);
-    cmax = Group<NT>::reduce(cmax, OpMax(), sc[g]);
-    // monotone bucket map b(c) = floor((c - cmin) * NB / range) computed as a 32.32 product
-    const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
-    const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
-    int ik[IPT], ib[IPT], io[IPT];
-    V iv[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const int s = t + j * NT;
-      const int key = K[s];
-      ik[j] = key;
-      iv[j] = H[s];
-      ib[j] = 0;
-      io[j] = 0;
-      if (key != kEmpty) {
-        const int b = (int)(((uint64_t)(uint32_t)(key - cmin) * mul) >> 32);
-        ib[j] = b;
-        io[j] = atomicAdd(&BC[b], 1);
+      for (int q = 0; q < IPT; ++q) {  // lane t owns slots [IPT*t, IPT*t + IPT): wide stores
+        K[IPT * t + q] = kEmpty;
+        H[IPT * t + q] = V(0);
       }
-    }
-    gsync<NT>();
-    {  // exclusive scan of bucket counts, PER contiguous buckets per thread
-      int loc[PER];
-      int sum = 0;
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int i = t * PER + q;
-        loc[q] = (i < NB) ? BC[i] : 0;
-        sum += loc[q];
+      wave_sync();
+      HashSink<T, V, true> sk{K, H};
+      for_row_products_pre<64, G, 4, true, V>(t, rp.a0, rp.a1, rp.d, bstart, blen, Aval, Bidx, Bval, sk, &st0);
+      rp.prefetch_desc(t, bstart, blen, Aval);
+      wave_sync();
+      const int cmin = wave_reduce_dpp(sk.cmin, OpMin());
+      const int cmax = wave_reduce_dpp(sk.cmax, OpMax());
+      int bad = sk.bad;
+      const int n = wave_sort_table<V, T>(t, K, H, cmin, cmax, bad);
+      if (rp.more) gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx, Bval);
+      const int64_t o0 = rp.c0 - cbase;
+      const int64_t nexp = rp.c1 - rp.c0;
+      const int nw = (int)min((int64_t)n, nexp);
+      for (int m = t; m < nw; m += 64) {
+        Cidx[o0 + m] = K[m];
+        Cval[o0 + m] = H[m];
       }
-      int total;
-      int pre = Group<NT>::exscan(sum, total, sc[g]);
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int i = t * PER + q;
-        if (i < NB) BC[i] = pre;
-        pre += loc[q];
+      bad = wave_reduce_dpp(bad, OpMax());
+      if (t == 0) {
+        if (bad) atomicOr(err, kErrTableFull);
+        if ((int64_t)n != nexp) atomicOr(err, kErrCountMismatch);
       }
-      if (t == 0) BC[NB] = total;
+      wave_sync();
+      if (!rp.advance(nrow2)) break;
     }
-    gsync<NT>();
-    int* perm = K;  // the table keys are in registers now; reuse their LDS
+    return;
+  } else {
+    int* BC = bc[g];
+    for (int64_t slot = beg + (int64_t)blockIdx.x * GPB + g; slot < end; slot += (int64_t)gridDim.x * GPB) {
+      const int row = rows[slot];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j)
-      if (ik[j] != kEmpty) perm[BC[ib[j]] + io[j]] = ik[j];
-    gsync<NT>();
-    int rk[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      rk[j] = 0;
-      if (ik[j] != kEmpty) {
-        const int s0 = BC[ib[j]], s1 = BC[ib[j] + 1];
-        int r = s0;
-        for (int m = s0; m < s1; ++m) r += (perm[m] < ik[j]);
-        rk[j] = r;
+      for (int s2 = t; s2 < T; s2 += NT) {
+        K[s2] = kEmpty;
+        H[s2] = V(0);
       }
-    }
-    gsync<NT>();
+      for (int s2 = t; s2 <= NB; s2 += NT) BC[s2] = 0;
+      gsync<NT>();
+      const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
+      HashSink<T, V, true> sk{K, H};
+      for_row_products<NT, G, 4, true, V>(t, a0, a1, bstart, blen, Aval, Bidx, Bval, sk);
+      gsync<NT>();
+      const int cmin = Group<NT>::reduce(sk.cmin, OpMin(), sc[g]);
+      const int cmax = Group<NT>::reduce(sk.cmax, OpMax(), sc[g]);
+      int bad = sk.bad;
+      const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
+      const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
+      int ik[IPT], ib[IPT], io[IPT];
+      V iv[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j)
-      if (ik[j] != kEmpty) {
-        K[rk[j]] = ik[j];
-        H[rk[j]] = iv[j];
+      for (int j = 0; j < IPT; ++j) {
+        const int s2 = t + j * NT;
+        const int key = K[s2];
+        ik[j] = key;
+        iv[j] = H[s2];
+        ib[j] = 0;
+        io[j] = 0;
+        if (key != kEmpty) {
+          const int b = (int)(((uint64_t)(uint32_t)(key - cmin) * mul) >> 32);
+          ib[j] = b;
+          io[j] = atomicAdd(&BC[b], 1);
+        }
       }
-    gsync<NT>();
-    const int n = BC[NB];
-    const int64_t o0 = Cptr[row] - cbase;
-    const int64_t nexp = Cptr[row + 1] - Cptr[row];
-    const int nw = (int)min((int64_t)n, nexp);
-    for (int m = t; m < nw; m += NT) {
-      Cidx[o0 + m] = K[m];
-      Cval[o0 + m] = H[m];
+      gsync<NT>();
+      {  // exclusive scan of bucket counts, PER contiguous buckets per thread
+        int loc[PER];
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int i = t * PER + q;
+          loc[q] = (i < NB) ? BC[i] : 0;
+          sum += loc[q];
+        }
+        int total;
+        int pre = Group<NT>::exscan(sum, total, sc[g]);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int i = t * PER + q;
+          if (i < NB) BC[i] = pre;
+          pre += loc[q];
+        }
+        if (t == 0) BC[NB] = total;
+      }
+      gsync<NT>();
+      int* perm = K;  // the table keys are in registers now; reuse their LDS
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (ik[j] != kEmpty) perm[BC[ib[j]] + io[j]] = ik[j];
+      gsync<NT>();
+      int rk[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        rk[j] = 0;
+        if (ik[j] != kEmpty) {
+          const int s0 = BC[ib[j]], s1 = BC[ib[j] + 1];
+          int r = s0;
+          for (int m = s0; m < s1; ++m) r += (perm[m] < ik[j]);
+          rk[j] = r;
+        }
+      }
+      gsync<NT>();
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (ik[j] != kEmpty) {
+          K[rk[j]] = ik[j];
+          H[rk[j]] = iv[j];
+        }
+      gsync<NT>();
+      const int n = BC[NB];
+      const int64_t o0 = Cptr[row] - cbase;
+      const int64_t nexp = Cptr[row + 1] - Cptr[row];
+      const int nw = (int)min((int64_t)n, nexp);
+      for (int m = t; m < nw; m += NT) {
+        Cidx[o0 + m] = K[m];
+        Cval[o0 + m] = H[m];
+      }
+      bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
+      if (t == 0) {
+        if (bad) atomicOr(err, kErrTableFull);
+        if ((int64_t)n != nexp) atomicOr(err, kErrCountMismatch);
+      }
+      gsync<NT>();
     }
-    bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
-    if (t == 0) {
-      if (bad) atomicOr(err, kErrTableFull);
-      if ((int64_t)n != nexp) atomicOr(err, kErrCountMismatch);
-    }
-    gsync<NT>();
   }
+}
+
+// Occupancy target of the one-wave-per-row numeric kernel (waves per SIMD); the compiler fits
+// its registers to it. The workgroup-per-row shapes are limited by LDS instead.
+#ifndef DBD_NUMERIC_WAVES_PER_EU
+#define DBD_NUMERIC_WAVES_PER_EU 4
+#endif
+
+#define DBD_NUMERIC_ARGS                                                                                     \
+  const int32_t *__restrict__ rows, const int64_t *__restrict__ pbeg, const int64_t *__restrict__ pend,     \
+      const int64_t *__restrict__ Aptr, const int64_t *__restrict__ bstart, const int32_t *__restrict__ blen, \
+      const V *__restrict__ Aval, const int32_t *__restrict__ Bidx, const V *__restrict__ Bval,              \
+      const int64_t *__restrict__ Cptr, int64_t cbase, int32_t *__restrict__ Cidx, V *__restrict__ Cval,      \
+      int *__restrict__ err
+
+template <typename V, int T, int NT, int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DBD_NUMERIC_WAVES_PER_EU))) void
+k_numeric_wave(DBD_NUMERIC_ARGS) {
+  numeric_hash_body<V, T, NT, G>(rows, pbeg, pend, Aptr, bstart, blen, Aval, Bidx, Bval, Cptr, cbase, Cidx, Cval,
+                                 err);
+}
+
+template <typename V, int T, int NT, int G>
+__global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
+  numeric_hash_body<V, T, NT, G>(rows, pbeg, pend, Aptr, bstart, blen, Aval, Bidx, Bval, Cptr, cbase, Cidx, Cval,
+                                 err);
 }
 
 // Long rows: dense LDS window of W columns (values + presence bitmap); emitted in column
@@ -958,6 +1390,9 @@ __global__ __launch_bounds__(NT) void k_backward_long(
 // ------------------------------------------------------------------------------------
 static int grid_for(int64_t nrows, int gpb, int grid) {
   if (grid > 0) return grid;
+  // one-wave-per-row kernels pipeline consecutive rows of a wave: give each wave several rows
+  // when the bin is large enough to still fill the GPU many times over
+  if (gpb > 1 && nrows >= (int64_t)gpb * 4 * 8192) gpb *= 4;
   int64_t g = (nrows + gpb - 1) / gpb;
   if (g > 1048576) g = 1048576;
   if (g < 1) g = 1;
@@ -1000,7 +1435,7 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
 #define X(T_, NT_, G_)                                                                                         \
   if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                               \
     hipLaunchKernelGGL((k_symbolic_hash<T_, NT_, G_>), gr, dim3(wg_of<NT_>()), 0, s, rows, pbeg, pend, A.ptr, \
-                       A.idx, B.ptr, B.idx, nnz, err);                                                        \
+                       A.bstart, A.blen, B.idx, nnz, err);                                                   \
     return;                                                                                                   \
   }
   DBD_SHAPES(X, 16)
@@ -1024,11 +1459,15 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
                        A.idx, Av, B.ptr, B.idx, Bv, Cptr, cbase, Cidx, Cval, cursor, err);
     return;
   }
-#define X(T_, NT_, G_)                                                                                            \
-  if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                                  \
-    hipLaunchKernelGGL((k_numeric_hash<V, T_, NT_, G_>), gr, dim3(wg_of<NT_>()), 0, s, rows, pbeg, pend, A.ptr, \
-                       A.idx, Av, B.ptr, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                              \
-    return;                                                                                                      \
+#define X(T_, NT_, G_)                                                                                           \
+  if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                                 \
+    if constexpr (NT_ == 64)                                                                                   \
+      hipLaunchKernelGGL((k_numeric_wave<V, T_, NT_, G_>), gr, dim3(256), 0, s, rows, pbeg, pend, A.ptr,       \
+                         A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                      \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_numeric_block<V, T_, NT_, G_>), gr, dim3(NT_), 0, s, rows, pbeg, pend, A.ptr,      \
+                         A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                      \
+    return;                                                                                                     \
   }
   DBD_SHAPES(X, 16)
   DBD_SHAPES(X, 64)
