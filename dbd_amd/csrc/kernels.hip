@@ -16,6 +16,7 @@
 // column range in LDS-sized windows with one cursor per A entry.
 #include "kernels.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace dbd {
@@ -29,6 +30,18 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wait until no vector memory operation of this wave is outstanding (s_waitcnt vmcnt(0); the
+// other counters at their maximum). Placed at the end of rarely taken branches that issue
+// loads, so the compiler's count of outstanding loads on the common path stays exact.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// 64-bit readlane of a wave-uniform lane index.
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // A "group" is NT threads that own one row. NT <= 64: a (sub-)wave, synchronised with
@@ -276,63 +289,57 @@ struct ADesc {
   V va;
 };
 
-// Lane `lane` loads the descriptor of A entry base + lane (empty past a1). These loads are
-// sequential in A's order (coalesced).
+// Lane `lane` loads the descriptor of A entry base + lane. These loads are sequential in A's
+// order (coalesced). Lanes past a1 load entry a1 - 1 instead, and the values are kept raw: the
+// consumers (gather_step, consume_step) take only entries below the chunk's count. Unconditional loads keep
+// the compiler's in-order count of outstanding loads exact (a branch around a load, or a select
+// on its result right after it, makes the wave wait for the load at that point). a1 >= 1 for
+// every row that reaches a kernel (such rows have products).
 template <bool WITHV, typename V>
 __device__ __forceinline__ ADesc<V> load_desc(int64_t base, int64_t a1, int lane, const int64_t* __restrict__ bstart,
                                               const int32_t* __restrict__ blen, const V* __restrict__ Aval) {
   const int64_t j = base + lane;
-  const bool ok = j < a1;
+  const int64_t jj = j < a1 ? j : (a1 > 0 ? a1 - 1 : 0);
   ADesc<V> d;
-  d.b0 = ok ? bstart[j] : 0;
-  d.len = ok ? blen[j] : 0;
-  d.va = (WITHV && ok) ? Aval[j] : V(0);
+  d.b0 = bstart[jj];
+  d.len = blen[jj];
+  d.va = WITHV ? Aval[jj] : V(0);
   return d;
 }
 
 // One gather step over a chunk of A entries whose descriptors are spread over the wave's
 // lanes: the wave's 64/G subgroups take entries through wave shuffles, U entries per step, and
-// issue the U B-row gathers (the first G entries of each B row). Only the gathered columns and
-// raw B values are kept; the products are formed when the step is consumed.
+// issue the U B-row gathers (the first G entries of each B row). Only the loaded columns and B
+// values are kept; they are masked and multiplied when the step is consumed.
 template <int U, typename V>
 struct Step {
-  int c[U];
-  V bv[U];
+  int c[U];   // B columns as loaded (masked when consumed)
+  V bv[U];    // B values as loaded
+  V pv[U];    // A value of each entry
 };
-
-template <int U, typename V>
-struct EntryRef {  // entry u of a step, as read back from the descriptors
-  int64_t pb[U];
-  int pl[U];
-  V pv[U];
-};
-
-template <int G, int U, bool WITHV, typename V>
-__device__ __forceinline__ void step_entries(EntryRef<U, V>& r, int lane, int e0, int cnt, const ADesc<V>& d) {
-  static_assert(64 % G == 0, "subgroups tile the wave");
-  constexpr int NG = 64 / G;
-  const int sg = lane / G;
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int e = e0 + u * NG + sg;
-    const int src = e < 64 ? e : 63;
-    r.pb[u] = __shfl(d.b0, src);
-    r.pl[u] = e < cnt ? __shfl(d.len, src) : 0;
-    r.pv[u] = WITHV ? __shfl(d.va, src) : V(0);
-  }
-}
 
 template <int G, int U, bool WITHV, typename V>
 __device__ __forceinline__ void gather_step(Step<U, V>& st, int lane, int e0, int cnt, const ADesc<V>& d,
                                             const int32_t* __restrict__ Bidx, const V* __restrict__ Bval) {
-  EntryRef<U, V> r;
-  step_entries<G, U, false, V>(r, lane, e0, cnt, d);
-  const int lg = lane % G;
+  static_assert(64 % G == 0, "subgroups tile the wave");
+  constexpr int NG = 64 / G;
+  const int sg = lane / G, lg = lane % G;
+  // unconditional loads: lanes past the B row re-read its last entry, entries without a B row
+  // read entry 0 (B has entries whenever a row reaches a kernel). The loaded values are kept
+  // raw: masking them here would make the wave wait for the loads right away.
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const bool has = lg < r.pl[u];
-    st.c[u] = has ? Bidx[r.pb[u] + lg] : -1;
-    st.bv[u] = (WITHV && has) ? Bval[r.pb[u] + lg] : V(0);
+    const int e = e0 + u * NG + sg;
+    const int src = e < 64 ? e : 63;
+    // shuffles run on all lanes (a lane reading from an inactive lane gets 0), then mask
+    const int64_t pb = __shfl(d.b0, src);
+    const int lv = __shfl(d.len, src);
+    const int pl = e < cnt ? lv : 0;
+    if constexpr (WITHV) st.pv[u] = __shfl(d.va, src);
+    else st.pv[u] = V(0);
+    const int64_t at = pl > 0 ? pb + min(lg, pl - 1) : 0;
+    st.c[u] = Bidx[at];
+    st.bv[u] = WITHV ? Bval[at] : V(0);
   }
 }
 
@@ -341,19 +348,34 @@ __device__ __forceinline__ void gather_step(Step<U, V>& st, int lane, int e0, in
 template <int G, int U, bool WITHV, typename V, typename F>
 __device__ __forceinline__ void consume_step(const Step<U, V>& st, int lane, int e0, int cnt, const ADesc<V>& d,
                                              const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F& f) {
-  EntryRef<U, V> r;
-  step_entries<G, U, WITHV, V>(r, lane, e0, cnt, d);
-  const int lg = lane % G;
+  constexpr int NG = 64 / G;
+  const int sg = lane / G, lg = lane % G;
+  int pl[U], c[U];
   V v[U];
+  bool tail = false;
 #pragma unroll
-  for (int u = 0; u < U; ++u) v[u] = WITHV ? r.pv[u] * st.bv[u] : V(0);
-  f.template batch<U>(st.c, v);
+  for (int u = 0; u < U; ++u) {
+    const int e = e0 + u * NG + sg;
+    const int lv = __shfl(d.len, e < 64 ? e : 63);  // on all lanes, then masked
+    pl[u] = e < cnt ? lv : 0;
+    const bool has = lg < pl[u];
+    c[u] = has ? st.c[u] : -1;
+    v[u] = (WITHV && has) ? st.pv[u] * st.bv[u] : V(0);
+    tail |= pl[u] > G;
+  }
+  f.template batch<U>(c, v);
+  if (__builtin_amdgcn_ballot_w64(tail)) {  // B rows longer than G: the rest of them, one by one
 #pragma unroll
-  for (int u = 0; u < U; ++u)
-    for (int q = G + lg; q < r.pl[u]; q += G) {
-      const int64_t p = r.pb[u] + q;
-      f.one(Bidx[p], WITHV ? r.pv[u] * Bval[p] : V(0));
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NG + sg;
+      const int64_t pb = __shfl(d.b0, e < 64 ? e : 63);
+      for (int q = G + lg; q < pl[u]; q += G) {
+        const int64_t p = pb + q;
+        f.one(Bidx[p], WITHV ? st.pv[u] * Bval[p] : V(0));
+      }
     }
+    wait_vmem();
+  }
 }
 
 // All products of `cnt` (<= 64) A entries of one chunk, steps of U entries per subgroup.
@@ -389,11 +411,17 @@ __device__ __forceinline__ void for_row_products_pre(int t, int64_t a0, int64_t 
       consume_step<G, U, WITHV, V>(*step0, lane, 0, cnt, first, Bidx, Bval, f);
       from = NG * U;
     }
-    walk_chunk<G, U, WITHV, V>(lane, cnt, first, Bidx, Bval, f, from);
+    if (from < cnt) {
+      walk_chunk<G, U, WITHV, V>(lane, cnt, first, Bidx, Bval, f, from);
+      if (step0) wait_vmem();
+    }
   }
-  for (base += NT; base < a1; base += NT) {
-    const ADesc<V> d = load_desc<WITHV, V>(base, a1, lane, bstart, blen, Aval);
-    walk_chunk<G, U, WITHV, V>(lane, (int)min((int64_t)64, a1 - base), d, Bidx, Bval, f);
+  if (base + NT < a1) {
+    for (base += NT; base < a1; base += NT) {
+      const ADesc<V> d = load_desc<WITHV, V>(base, a1, lane, bstart, blen, Aval);
+      walk_chunk<G, U, WITHV, V>(lane, (int)min((int64_t)64, a1 - base), d, Bidx, Bval, f);
+    }
+    if (step0) wait_vmem();
   }
 }
 
@@ -405,18 +433,55 @@ __device__ __forceinline__ void for_row_products(int t, int64_t a0, int64_t a1, 
   for_row_products_pre<NT, G, U, WITHV, V>(t, a0, a1, first, bstart, blen, Aval, Bidx, Bval, f);
 }
 
-// Rows handled by one wave, software-pipelined: the row id two ahead, the row pointers of the
-// next row, and (after the current row's products are in LDS) the next row's A descriptors
-// are loaded early, so only the B gathers stay on each row's critical path.
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). When the grid
+// is a multiple of 8, virtual block ids are regrouped so that consecutive ids (consecutive bin
+// slots, i.e. nearby rows) run on the same XCD.
+__device__ __forceinline__ int64_t xcd_block() {
+  const uint32_t b = blockIdx.x, n = gridDim.x;
+  if (n % 8 != 0) return b;
+  return (int64_t)(b % 8) * (n / 8) + b / 8;
+}
+
+// Rows handled by one wave, software-pipelined. The wave's rows are the bin slots s0, s0 +
+// stride, ...; their ids and row pointers are loaded 64 rows at a time with vector loads (lane
+// L holds row L of the batch) and read out with readlane, so no scalar loads are outstanding
+// while the wave waits on LDS (scalar loads and LDS share one counter). The next row's A
+// descriptors are issued at the top of each iteration and its first B gathers after the
+// current row's products are in LDS, so each wait is for loads issued a good while earlier.
 template <bool WITHV, typename V>
 struct RowPipe {
-  int64_t slot, stride, end;
-  int row, nrow;
-  int64_t a0, a1, c0, c1;     // current row: A range and output range
-  int64_t na0, na1, nc0, nc1; // next row
-  ADesc<V> d, nd;
-  bool more;
+  int64_t slot, stride, end;   // slot of the current row (wave-uniform)
+  int kb;                      // batch index of the next row (wave-uniform)
+  int bR;                      // batch: row ids (per lane)
+  int64_t bA0, bA1, bC0, bC1;  // batch: row pointers (per lane)
+  int row, row1;               // current / next row (wave-uniform)
+  int64_t a0, a1, c0, c1;      // current row: A range and output range
+  int64_t na0, na1, nc0, nc1;  // next row
+  ADesc<V> d, nd;              // descriptors of the current / next row
+  bool more;                   // a next row exists
 
+  __device__ __forceinline__ void load_batch(int64_t s0, int lane, const int32_t* __restrict__ rows,
+                                             const int64_t* __restrict__ Aptr, const int64_t* __restrict__ Cptr) {
+    int64_t ls = s0 + (int64_t)lane * stride;
+    if (ls >= end) ls = end - 1;  // past the bin: repeat its last row (never used)
+    const int r = rows[ls];
+    bR = r;
+    bA0 = Aptr[r];
+    bA1 = Aptr[r + 1];
+    if (Cptr) {
+      bC0 = Cptr[r];
+      bC1 = Cptr[r + 1];
+    } else {
+      bC0 = bC1 = 0;
+    }
+  }
+  __device__ __forceinline__ void extract(int j, int& r, int64_t& x0, int64_t& x1, int64_t& y0, int64_t& y1) const {
+    r = __builtin_amdgcn_readlane(bR, j);
+    x0 = readlane64(bA0, j);
+    x1 = readlane64(bA1, j);
+    y0 = readlane64(bC0, j);
+    y1 = readlane64(bC1, j);
+  }
   __device__ __forceinline__ void start(int64_t s0, int64_t st, int64_t e, const int32_t* __restrict__ rows,
                                         const int64_t* __restrict__ Aptr, const int64_t* __restrict__ Cptr, int lane,
                                         const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
@@ -424,42 +489,33 @@ struct RowPipe {
     slot = s0;
     stride = st;
     end = e;
-    row = rows[slot];
-    a0 = Aptr[row];
-    a1 = Aptr[row + 1];
-    if (Cptr) {
-      c0 = Cptr[row];
-      c1 = Cptr[row + 1];
-    }
+    load_batch(s0, lane, rows, Aptr, Cptr);
+    kb = 0;
+    extract(0, row, a0, a1, c0, c1);
     d = load_desc<WITHV, V>(a0, a1, lane, bstart, blen, Aval);
-    nrow = (slot + stride < end) ? rows[slot + stride] : 0;
   }
-  // top of an iteration: row pointers of the next row, id of the one after
-  __device__ __forceinline__ int prefetch_ptrs(const int32_t* __restrict__ rows, const int64_t* __restrict__ Aptr,
-                                               const int64_t* __restrict__ Cptr) {
+  // top of an iteration: pointers (a new batch every 64 rows) and descriptors of the next row;
+  // past the last row the current row is repeated, so the loads stay unconditional
+  __device__ __forceinline__ void prefetch(int lane, const int32_t* __restrict__ rows,
+                                           const int64_t* __restrict__ Aptr, const int64_t* __restrict__ Cptr,
+                                           const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
+                                           const V* __restrict__ Aval) {
     more = slot + stride < end;
-    int nrow2 = 0;
     if (more) {
-      na0 = Aptr[nrow];
-      na1 = Aptr[nrow + 1];
-      if (Cptr) {
-        nc0 = Cptr[nrow];
-        nc1 = Cptr[nrow + 1];
+      ++kb;
+      if (kb == 64) {
+        load_batch(slot + stride, lane, rows, Aptr, Cptr);
+        kb = 0;
+        wait_vmem();
       }
-      if (slot + 2 * stride < end) nrow2 = rows[slot + 2 * stride];
     }
-    return nrow2;
+    extract(kb, row1, na0, na1, nc0, nc1);
+    nd = load_desc<WITHV, V>(na0, na1, lane, bstart, blen, Aval);
   }
-  // after the current row's products: the next row's descriptors
-  __device__ __forceinline__ void prefetch_desc(int lane, const int64_t* __restrict__ bstart,
-                                                const int32_t* __restrict__ blen, const V* __restrict__ Aval) {
-    if (more) nd = load_desc<WITHV, V>(na0, na1, lane, bstart, blen, Aval);
-  }
-  __device__ __forceinline__ bool advance(int nrow2) {
+  __device__ __forceinline__ bool advance() {
     if (!more) return false;
     slot += stride;
-    row = nrow;
-    nrow = nrow2;
+    row = row1;
     a0 = na0;
     a1 = na1;
     c0 = nc0;
@@ -719,27 +775,33 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
   int* K = hk[g];
   const int64_t beg = *pbeg, end = *pend;
   if constexpr (NT == 64) {
-    const int64_t s0 = beg + (int64_t)blockIdx.x * GPB + g;
+    const int64_t s0 = beg + xcd_block() * GPB + g;
     if (s0 >= end) return;
     RowPipe<false, float> rp;
     rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, nullptr, t, bstart, blen, (const float*)nullptr);
+    Step<4, float> st0;  // first gather step of the current row, issued during the previous row
+    gather_step<G, 4, false, float>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx,
+                                    (const float*)nullptr);
     for (;;) {
-      const int nrow2 = rp.prefetch_ptrs(rows, Aptr, nullptr);
+      rp.prefetch(t, rows, Aptr, nullptr, bstart, blen, (const float*)nullptr);
 #pragma unroll
       for (int q = 0; q < T / 64; ++q) K[(T / 64) * t + q] = kEmpty;
       wave_sync();
       HashSink<T, float, false, false> sk{K, nullptr};
       for_row_products_pre<64, G, 4, false, float>(t, rp.a0, rp.a1, rp.d, bstart, blen, (const float*)nullptr, Bidx,
-                                                   (const float*)nullptr, sk);
-      rp.prefetch_desc(t, bstart, blen, (const float*)nullptr);
+                                                   (const float*)nullptr, sk, &st0);
       const int cnt = wave_reduce_dpp(sk.created, OpAdd());
       const int bad = wave_reduce_dpp(sk.bad, OpMax());
       if (t == 0) {
         nnz[rp.row] = cnt;
         if (bad) atomicOr(err, kErrTableFull);
       }
+      // the next row's first gathers (past the last row: the current row again, unused), issued
+      // after the store so that waiting for them never waits for a younger store
+      gather_step<G, 4, false, float>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx,
+                                      (const float*)nullptr);
       wave_sync();
-      if (!rp.advance(nrow2)) break;
+      if (!rp.advance()) break;
     }
     return;
   }
@@ -840,62 +902,39 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 // numeric: accumulate in an LDS hash, sort by a monotone bucket map + in-bucket rank,
 // emit coalesced
 // ------------------------------------------------------------------------------------
-// Sort the live entries of a one-wave row table (T slots, at most T/2 live) in place and
-// return their count: compact live slots (a lane's write position counts the live slots before
-// its own, so it never passes an unread slot), bucket them with the monotone map
-// b(c) = floor((c - cmin) * NB / range), scan the bucket counts, and rank each key inside its
-// bucket. Counters and the permutation live in the table's upper halves.
-template <typename V, int T>
-__device__ __forceinline__ int wave_sort_table(int t, int* K, V* H, int cmin, int cmax, int& bad) {
+// Rank the live entries of a one-wave row table (T slots, at most T/2 live) in column order
+// without moving them. Each lane takes the keys of its IPT slots into registers; the key table
+// is then reused for the bucket counters (lower half) and the bucket permutation (upper half).
+// b(c) = floor((c - cmin) * NB / range) is monotone, so a key's rank is the start of its bucket
+// plus the number of smaller keys in the bucket. rk[q] = rank of slot IPT*t + q, -1 if empty.
+// Returns the number of live entries (more than T/2 means the table was overfilled; ranks past
+// T/2 are then not exact and the caller reports the error).
+template <int T>
+__device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax, int (&kk)[T / 64],
+                                               int (&rk)[T / 64]) {
   constexpr int IPT = T / 64;
-  constexpr int NB = T / 2 - 1;
-  constexpr int PER = (NB + 63) / 64;
-  constexpr int IQ = (T / 2 + 63) / 64;
+  constexpr int H2 = T / 2;
+  constexpr int NB = H2 - 1;                 // buckets; BC[NB] receives the total
+  constexpr int PER = (H2 + 63) / 64;        // counters per lane in the scan
   const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
   const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
-  int kk[IPT];
-  V vv[IPT];
-  int live = 0;
+  int* BC = K;
+  int* perm = K + H2;
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) kk[q] = K[IPT * t + q];
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < PER; ++q)
+    if (t * PER + q < H2) BC[t * PER + q] = 0;
+  wave_sync();
+  int io[IPT];
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
-    kk[q] = K[IPT * t + q];
-    vv[q] = H[IPT * t + q];
-    live += (kk[q] != kEmpty);
-  }
-  int n;
-  int pos = wave_exscan_small<4>(live, n);  // live <= IPT <= 8
-#pragma unroll
-  for (int q = 0; q < IPT; ++q)
-    if (kk[q] != kEmpty) {
-      K[pos] = kk[q];
-      H[pos] = vv[q];
-      ++pos;
-    }
-  wave_sync();
-  int* BC = (int*)(H + T / 2);
-  int* perm = K + T / 2;
-  bad |= (n > T / 2);
-  int ik[IQ], ib[IQ], io[IQ], rk[IQ];
-  V iv[IQ];
-#pragma unroll
-  for (int q = 0; q < IQ; ++q) {
-    const int i = t + 64 * q;
-    ik[q] = (i < n && i < T / 2) ? K[i] : kEmpty;
-    iv[q] = (i < n && i < T / 2) ? H[i] : V(0);
-  }
-  for (int s2 = t; s2 <= NB; s2 += 64) BC[s2] = 0;
-  wave_sync();
-#pragma unroll
-  for (int q = 0; q < IQ; ++q) {
-    ib[q] = 0;
     io[q] = 0;
-    if (ik[q] != kEmpty) {
-      const int b = (int)(((uint64_t)(uint32_t)(ik[q] - cmin) * mul) >> 32);
-      ib[q] = b;
-      io[q] = atomicAdd(&BC[b], 1);
-    }
+    if (kk[q] != kEmpty) io[q] = atomicAdd(&BC[(int)(((uint64_t)(uint32_t)(kk[q] - cmin) * mul) >> 32)], 1);
   }
   wave_sync();
+  int total;
   {
     int loc[PER];
     int sum = 0;
@@ -905,8 +944,7 @@ __device__ __forceinline__ int wave_sort_table(int t, int* K, V* H, int cmin, in
       loc[q] = (i < NB) ? BC[i] : 0;
       sum += loc[q];
     }
-    int total;
-    int pre = wave_exscan_small<__builtin_ctz(T / 2) + 1>(sum, total);  // sum <= T/2
+    int pre = wave_exscan_small<__builtin_ctz(T) + 1>(sum, total);  // sum <= T
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int i = t * PER + q;
@@ -917,28 +955,24 @@ __device__ __forceinline__ int wave_sort_table(int t, int* K, V* H, int cmin, in
   }
   wave_sync();
 #pragma unroll
-  for (int q = 0; q < IQ; ++q)
-    if (ik[q] != kEmpty) perm[BC[ib[q]] + io[q]] = ik[q];
+  for (int q = 0; q < IPT; ++q)
+    if (kk[q] != kEmpty) {
+      const int pos = BC[(int)(((uint64_t)(uint32_t)(kk[q] - cmin) * mul) >> 32)] + io[q];
+      if (pos < H2) perm[pos] = kk[q];
+    }
   wave_sync();
 #pragma unroll
-  for (int q = 0; q < IQ; ++q) {
-    rk[q] = 0;
-    if (ik[q] != kEmpty) {
-      const int s0 = BC[ib[q]], s1 = BC[ib[q] + 1];
+  for (int q = 0; q < IPT; ++q) {
+    rk[q] = -1;
+    if (kk[q] != kEmpty) {
+      const int b = (int)(((uint64_t)(uint32_t)(kk[q] - cmin) * mul) >> 32);
+      const int s0 = BC[b], s1 = min(BC[b + 1], H2);
       int r = s0;
-      for (int m = s0; m < s1; ++m) r += (perm[m] < ik[q]);
+      for (int m = s0; m < s1; ++m) r += (perm[m] < kk[q]);
       rk[q] = r;
     }
   }
-  wave_sync();
-#pragma unroll
-  for (int q = 0; q < IQ; ++q)
-    if (ik[q] != kEmpty) {
-      K[rk[q]] = ik[q];
-      H[rk[q]] = iv[q];
-    }
-  wave_sync();
-  return n > T / 2 ? T / 2 : n;
+  return total;
 }
 
 template <typename V, int T, int NT, int G>
@@ -965,19 +999,16 @@ __device__ __forceinline__ void numeric_hash_body(
   V* H = hv[g];
   const int64_t beg = *pbeg, end = *pend;
   if constexpr (WAVE) {
-    // one wave per row, rows software-pipelined (RowPipe); the table is sorted by
-    // wave_sort_table while the next row's descriptors are in flight
-    const int64_t s0 = beg + (int64_t)blockIdx.x * GPB + g;
+    // one wave per row, rows software-pipelined (RowPipe); the table is ranked by
+    // wave_rank_table while the next row's descriptors are in flight
+    const int64_t s0 = beg + xcd_block() * GPB + g;
     if (s0 >= end) return;
     RowPipe<true, V> rp;
     rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, Cptr, t, bstart, blen, Aval);
-    // the first gather step of each row is issued before the previous row's output stores: on
-    // this architecture stores share the load counter, so a wait for gathers issued after the
-    // stores would also wait for the stores
-    Step<4, V> st0;
+    Step<4, V> st0;  // first gather step of the current row, issued during the previous row
     gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx, Bval);
     for (;;) {
-      const int nrow2 = rp.prefetch_ptrs(rows, Aptr, Cptr);
+      rp.prefetch(t, rows, Aptr, Cptr, bstart, blen, Aval);
 #pragma unroll
       for (int q = 0; q < IPT; ++q) {  // lane t owns slots [IPT*t, IPT*t + IPT): wide stores
         K[IPT * t + q] = kEmpty;
@@ -986,19 +1017,26 @@ __device__ __forceinline__ void numeric_hash_body(
       wave_sync();
       HashSink<T, V, true> sk{K, H};
       for_row_products_pre<64, G, 4, true, V>(t, rp.a0, rp.a1, rp.d, bstart, blen, Aval, Bidx, Bval, sk, &st0);
-      rp.prefetch_desc(t, bstart, blen, Aval);
       wave_sync();
       const int cmin = wave_reduce_dpp(sk.cmin, OpMin());
       const int cmax = wave_reduce_dpp(sk.cmax, OpMax());
       int bad = sk.bad;
-      const int n = wave_sort_table<V, T>(t, K, H, cmin, cmax, bad);
-      if (rp.more) gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx, Bval);
+      int kk[IPT], rk[IPT];
+      const int n = wave_rank_table<T>(t, K, cmin, cmax, kk, rk);
+      bad |= (n > T / 2);
+      // the next row's first gathers (past the last row: the current row again, unused)
+      gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx, Bval);
+      // each live slot goes straight to its output position (the row's output is a few KiB;
+      // the partial lines of neighbouring lanes merge in L2)
       const int64_t o0 = rp.c0 - cbase;
       const int64_t nexp = rp.c1 - rp.c0;
-      const int nw = (int)min((int64_t)n, nexp);
-      for (int m = t; m < nw; m += 64) {
-        Cidx[o0 + m] = K[m];
-        Cval[o0 + m] = H[m];
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) {
+        const V vq = H[IPT * t + q];
+        if (rk[q] >= 0 && rk[q] < nexp) {
+          Cidx[o0 + rk[q]] = kk[q];
+          Cval[o0 + rk[q]] = vq;
+        }
       }
       bad = wave_reduce_dpp(bad, OpMax());
       if (t == 0) {
@@ -1006,7 +1044,7 @@ __device__ __forceinline__ void numeric_hash_body(
         if ((int64_t)n != nexp) atomicOr(err, kErrCountMismatch);
       }
       wave_sync();
-      if (!rp.advance(nrow2)) break;
+      if (!rp.advance()) break;
     }
     return;
   } else {
@@ -1238,20 +1276,27 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
 // ------------------------------------------------------------------------------------
 // backward: dA_ik = sum_j dC_ij B_kj ; dB_kj += A_ik dC_ij  (pattern of C from forward)
 // ------------------------------------------------------------------------------------
+// Per C row: dC_i goes into an LDS table keyed by column; then every A entry of the row walks
+// its B row (descriptor chunks, U gathers in flight per subgroup as in the forward kernels),
+// looks each product column up, reduces dA_ik over its subgroup and adds A_ik * dC_ij into dB.
 template <typename V, int T, int NT, int G>
 __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
-    const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx, const V* __restrict__ Aval,
-    const int64_t* __restrict__ Bptr, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+    const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
+    const V* __restrict__ Aval, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
     const int64_t* __restrict__ Cptr, const int32_t* __restrict__ Cidx, const V* __restrict__ dC,
     V* __restrict__ dA, V* __restrict__ dB, int* __restrict__ err) {
   constexpr int WG = wg_of<NT>();
   constexpr int GPB = WG / NT;
-  static_assert(NT % G == 0 && T % NT == 0, "shape");
+  constexpr int NG = 64 / G;
+  constexpr int U = 4;
+  static_assert(NT % 64 == 0 && NT % G == 0 && T % NT == 0, "shape");
   __shared__ int hk[GPB][T];
   __shared__ V hv[GPB][T];
   __shared__ int sc[GPB][NT > 64 ? NT / 64 : 1];
-  const int g = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int t = threadIdx.x % NT;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);
+  const int lane = t & 63, wv = t >> 6, sg = lane / G, lg = lane % G;
   int* K = hk[g];
   V* H = hv[g];
   const int64_t beg = *pbeg, end = *pend;
@@ -1276,24 +1321,51 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
     }
     gsync<NT>();
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
-    for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
-      const int k = Aidx[ka];
-      const V va = Aval[ka];
-      const int64_t b0 = Bptr[k], b1 = Bptr[k + 1];
-      V acc = V(0);
-      for (int64_t p = b0 + (t % G); p < b1; p += G) {
-        const int s = hash_find<T>(K, Bidx[p]);
-        if (s < 0) {
-          bad = 1;
-          continue;
-        }
-        const V gv = H[s];
-        acc += gv * Bval[p];
-        if (dB) atomicAdd(&dB[p], va * gv);
-      }
+    for (int64_t base = a0 + 64 * wv; base < a1; base += NT) {  // wave-uniform
+      const int cnt = (int)min((int64_t)64, a1 - base);
+      const ADesc<V> d = load_desc<true, V>(base, a1, lane, bstart, blen, Aval);
+      for (int e0 = 0; e0 < cnt; e0 += NG * U) {
+        Step<U, V> st;
+        gather_step<G, U, true, V>(st, lane, e0, cnt, d, Bidx, Bval);
+        V acc[U];
 #pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
-      if (dA && (t % G) == 0) dA[ka] = acc;
+        for (int u = 0; u < U; ++u) {
+          const int e = e0 + u * NG + sg;
+          const int src = e < 64 ? e : 63;
+          const int lv = __shfl(d.len, src);  // on all lanes, then masked
+          const int pl = e < cnt ? lv : 0;
+          const int64_t pb = __shfl(d.b0, src);
+          acc[u] = V(0);
+          if (lg < pl) {
+            const int s = hash_find<T>(K, st.c[u]);
+            if (s < 0) {
+              bad = 1;  // product column absent from C's row: C does not belong to these operands
+            } else {
+              const V gv = H[s];
+              acc[u] = gv * st.bv[u];
+              if (dB) atomicAdd(&dB[pb + lg], st.pv[u] * gv);
+            }
+          }
+          for (int q = G + lg; q < pl; q += G) {
+            const int64_t p = pb + q;
+            const int s = hash_find<T>(K, Bidx[p]);
+            if (s < 0) {
+              bad = 1;
+              continue;
+            }
+            const V gv = H[s];
+            acc[u] += gv * Bval[p];
+            if (dB) atomicAdd(&dB[p], st.pv[u] * gv);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int o = G / 2; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o, G);
+          const int e = e0 + u * NG + sg;
+          if (dA && lg == 0 && e < cnt) dA[base + e] = acc[u];
+        }
+      }
     }
     bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
     if (t == 0 && bad) atomicOr(err, kErrMissingKey);
@@ -1388,12 +1460,24 @@ __global__ __launch_bounds__(NT) void k_backward_long(
 // ------------------------------------------------------------------------------------
 // dispatch tables
 // ------------------------------------------------------------------------------------
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
 static int grid_for(int64_t nrows, int gpb, int grid) {
   if (grid > 0) return grid;
-  // one-wave-per-row kernels pipeline consecutive rows of a wave: give each wave several rows
-  // when the bin is large enough to still fill the GPU many times over
-  if (gpb > 1 && nrows >= (int64_t)gpb * 4 * 8192) gpb *= 4;
   int64_t g = (nrows + gpb - 1) / gpb;
+  // one-wave-per-row kernels walk many rows per wave (their row pipeline amortises over the
+  // rows): twice the workgroups that are resident at 4 per CU, a multiple of the 8 XCDs
+  if (gpb > 1) g = std::min<int64_t>(g, (int64_t)device_cus() * 8);
   if (g > 1048576) g = 1048576;
   if (g < 1) g = 1;
   return (int)g;
@@ -1503,7 +1587,7 @@ static void launch_backward_t(const KernelShape& k, const int32_t* rows, const i
 #define X(T_, NT_, G_)                                                                                             \
   if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                                   \
     hipLaunchKernelGGL((k_backward_hash<V, T_, NT_, G_>), gr, dim3(wg_of<NT_>()), 0, s, rows, pbeg, pend, A.ptr, \
-                       A.idx, Av, B.ptr, B.idx, Bv, Cptr, Cidx, dC, dA, dB, err);                                \
+                       A.bstart, A.blen, Av, B.idx, Bv, Cptr, Cidx, dC, dA, dB, err);                            \
     return;                                                                                                       \
   }
   DBD_SHAPES(X, 16)
