@@ -347,7 +347,7 @@ dbd_status plan(dbd_ctx* c) {
   CHK(c->offs_num.ensure(8 * (ncnt + 1)));
   CHK(c->tmp.ensure(8 * (size_t)scan_tmp_elems((int64_t)ncnt + nr + 1)));
   CHK(c->blk.ensure(8 * (size_t)(std::max(c->nb, (int64_t)kMaxCats) + 2)));
-  CHK(c->err.ensure(sizeof(int)));
+  CHK(c->err.ensure(64));  // error word, then a sink the numeric kernels write degenerate rows to
   CHK(c->bdesc_s.ensure(8 * (size_t)(c->A.nnz + 1)));
   CHK(c->bdesc_l.ensure(4 * (size_t)(c->A.nnz + 1)));
   c->A.bstart = c->bdesc_s.as<int64_t>();

@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
+#include <map>
 
 namespace dbd {
 
@@ -115,6 +117,9 @@ struct OpMax {
   template <typename T>
   __device__ T operator()(T a, T b) const { return a > b ? a : b; }
 };
+struct OpOr {
+  __device__ int operator()(int a, int b) const { return a | b; }
+};
 struct OpAdd {
   template <typename T>
   __device__ T operator()(T a, T b) const { return a + b; }
@@ -133,19 +138,18 @@ __device__ __forceinline__ int wave_reduce_dpp(int x, Op op) {
   return op(op(r0, r1), op(r2, r3));
 }
 
-// Exclusive prefix over the wave of a small non-negative int (< 2^NBITS), bit by bit with
-// ballots and mbcnt; no LDS traffic. `total` receives the wave sum.
-template <int NBITS>
-__device__ __forceinline__ int wave_exscan_small(int x, int& total) {
-  int pre = 0, tot = 0;
-#pragma unroll
-  for (int b = 0; b < NBITS; ++b) {
-    const uint64_t m = __ballot((x >> b) & 1);
-    pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
-    tot += __popcll(m) << b;
-  }
-  total = tot;
-  return pre;
+// Inclusive prefix sum over the wave with DPP: shifts of 1-3, 4 and 8 lanes inside each row of
+// 16 lanes, then the row totals through the row broadcasts of lanes 15 and 31. All 64 lanes
+// must be active.
+__device__ __forceinline__ int wave_incl_scan_dpp(int x) {
+  int s = x + __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  s += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);         // row_shr:2
+  s += __builtin_amdgcn_update_dpp(0, x, 0x113, 0xF, 0xF, false);         // row_shr:3
+  s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xF, 0xE, false);         // row_shr:4, lanes 4-15
+  s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xF, 0xC, false);         // row_shr:8, lanes 8-15
+  s += __builtin_amdgcn_update_dpp(0, s, 0x142, 0xA, 0xF, false);         // row_bcast:15 -> rows 1, 3
+  s += __builtin_amdgcn_update_dpp(0, s, 0x143, 0xC, 0xF, false);         // row_bcast:31 -> rows 2, 3
+  return s;
 }
 
 // Mask of the G lanes (within this wave) that share this lane's subgroup.
@@ -315,7 +319,6 @@ template <int U, typename V>
 struct Step {
   int c[U];   // B columns as loaded (masked when consumed)
   V bv[U];    // B values as loaded
-  V pv[U];    // A value of each entry
 };
 
 template <int G, int U, bool WITHV, typename V>
@@ -335,8 +338,6 @@ __device__ __forceinline__ void gather_step(Step<U, V>& st, int lane, int e0, in
     const int64_t pb = __shfl(d.b0, src);
     const int lv = __shfl(d.len, src);
     const int pl = e < cnt ? lv : 0;
-    if constexpr (WITHV) st.pv[u] = __shfl(d.va, src);
-    else st.pv[u] = V(0);
     const int64_t at = pl > 0 ? pb + min(lg, pl - 1) : 0;
     st.c[u] = Bidx[at];
     st.bv[u] = WITHV ? Bval[at] : V(0);
@@ -351,16 +352,17 @@ __device__ __forceinline__ void consume_step(const Step<U, V>& st, int lane, int
   constexpr int NG = 64 / G;
   const int sg = lane / G, lg = lane % G;
   int pl[U], c[U];
-  V v[U];
+  V v[U], pv[U];
   bool tail = false;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int e = e0 + u * NG + sg;
     const int lv = __shfl(d.len, e < 64 ? e : 63);  // on all lanes, then masked
+    pv[u] = WITHV ? __shfl(d.va, e < 64 ? e : 63) : V(0);
     pl[u] = e < cnt ? lv : 0;
     const bool has = lg < pl[u];
     c[u] = has ? st.c[u] : -1;
-    v[u] = (WITHV && has) ? st.pv[u] * st.bv[u] : V(0);
+    v[u] = (WITHV && has) ? pv[u] * st.bv[u] : V(0);
     tail |= pl[u] > G;
   }
   f.template batch<U>(c, v);
@@ -371,7 +373,7 @@ __device__ __forceinline__ void consume_step(const Step<U, V>& st, int lane, int
       const int64_t pb = __shfl(d.b0, e < 64 ? e : 63);
       for (int q = G + lg; q < pl[u]; q += G) {
         const int64_t p = pb + q;
-        f.one(Bidx[p], WITHV ? st.pv[u] * Bval[p] : V(0));
+        f.one(Bidx[p], WITHV ? pv[u] * Bval[p] : V(0));
       }
     }
     wait_vmem();
@@ -782,6 +784,9 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
     Step<4, float> st0;  // first gather step of the current row, issued during the previous row
     gather_step<G, 4, false, float>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx,
                                     (const float*)nullptr);
+    // row counts are collected one row per lane and written 64 rows at a time; error bits of
+    // all rows are reported once after the loop
+    int crow = 0, ccnt = 0, jn = 0, flags = 0;
     for (;;) {
       rp.prefetch(t, rows, Aptr, nullptr, bstart, blen, (const float*)nullptr);
 #pragma unroll
@@ -790,19 +795,23 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
       HashSink<T, float, false, false> sk{K, nullptr};
       for_row_products_pre<64, G, 4, false, float>(t, rp.a0, rp.a1, rp.d, bstart, blen, (const float*)nullptr, Bidx,
                                                    (const float*)nullptr, sk, &st0);
-      const int cnt = wave_reduce_dpp(sk.created, OpAdd());
-      const int bad = wave_reduce_dpp(sk.bad, OpMax());
-      if (t == 0) {
-        nnz[rp.row] = cnt;
-        if (bad) atomicOr(err, kErrTableFull);
-      }
-      // the next row's first gathers (past the last row: the current row again, unused), issued
-      // after the store so that waiting for them never waits for a younger store
+      // the next row's first gathers (past the last row: the current row again, unused)
       gather_step<G, 4, false, float>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx,
                                       (const float*)nullptr);
+      const int cnt = wave_reduce_dpp(sk.created, OpAdd());
+      flags |= sk.bad ? kErrTableFull : 0;
+      crow = (t == jn) ? rp.row : crow;
+      ccnt = (t == jn) ? cnt : ccnt;
+      ++jn;
+      if (jn == 64 || !rp.more) {
+        if (t < jn) nnz[crow] = ccnt;
+        jn = 0;
+      }
       wave_sync();
       if (!rp.advance()) break;
     }
+    flags = wave_reduce_dpp(flags, OpOr());
+    if (t == 0 && flags) atomicOr(err, flags);
     return;
   }
   for (int64_t slot = beg + (int64_t)blockIdx.x * GPB + g; slot < end; slot += (int64_t)gridDim.x * GPB) {
@@ -905,19 +914,24 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 // Rank the live entries of a one-wave row table (T slots, at most T/2 live) in column order
 // without moving them. Each lane takes the keys of its IPT slots into registers; the key table
 // is then reused for the bucket counters (lower half) and the bucket permutation (upper half).
-// b(c) = floor((c - cmin) * NB / range) is monotone, so a key's rank is the start of its bucket
-// plus the number of smaller keys in the bucket. rk[q] = rank of slot IPT*t + q, -1 if empty.
-// Returns the number of live entries (more than T/2 means the table was overfilled; ranks past
-// T/2 are then not exact and the caller reports the error).
+// The bucket map b(x) = floor(x * mul / 2^32), x = c - cmin (or b = x when the column range
+// fits the buckets) is monotone, so a key's rank is the start of its bucket plus the number of
+// smaller keys in the bucket; mul only has to keep b below NB, so it is formed in single
+// precision and rounded down. LDS reads of all slots are issued before their results are used.
+// rk[q] = rank of slot IPT*t + q, -1 if empty. Returns the number of live entries (more than
+// T/2 means the table was overfilled; ranks are then not exact and the caller reports it).
 template <int T>
 __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax, int (&kk)[T / 64],
                                                int (&rk)[T / 64]) {
   constexpr int IPT = T / 64;
   constexpr int H2 = T / 2;
-  constexpr int NB = H2 - 1;                 // buckets; BC[NB] receives the total
-  constexpr int PER = (H2 + 63) / 64;        // counters per lane in the scan
+  constexpr int NB = H2 - 1;           // buckets; BC[NB] receives the total
+  constexpr int PER = (H2 + 63) / 64;  // counters per lane in the scan
+  static_assert(NB < 65536 && T <= 65536, "bucket and in-bucket index share one int");
   const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
-  const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
+  const bool direct = range <= (uint32_t)NB;
+  const uint32_t mul =
+      direct ? 0u : (uint32_t)((float)NB * 4294967296.0f / (float)range * 0.99999f);
   int* BC = K;
   int* perm = K + H2;
 #pragma unroll
@@ -927,11 +941,13 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
   for (int q = 0; q < PER; ++q)
     if (t * PER + q < H2) BC[t * PER + q] = 0;
   wave_sync();
-  int io[IPT];
+  int bi[IPT];  // bucket | (arrival order in the bucket) << 16
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
-    io[q] = 0;
-    if (kk[q] != kEmpty) io[q] = atomicAdd(&BC[(int)(((uint64_t)(uint32_t)(kk[q] - cmin) * mul) >> 32)], 1);
+    const uint32_t x = (uint32_t)(kk[q] - cmin);
+    const int b = direct ? (int)x : (int)__umulhi(x, mul);
+    bi[q] = b;
+    if (kk[q] != kEmpty) bi[q] = b | (atomicAdd(&BC[b], 1) << 16);
   }
   wave_sync();
   int total;
@@ -944,7 +960,9 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
       loc[q] = (i < NB) ? BC[i] : 0;
       sum += loc[q];
     }
-    int pre = wave_exscan_small<__builtin_ctz(T) + 1>(sum, total);  // sum <= T
+    const int incl = wave_incl_scan_dpp(sum);
+    total = __builtin_amdgcn_readlane(incl, 63);
+    int pre = incl - sum;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int i = t * PER + q;
@@ -954,23 +972,37 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
     if (t == 0) BC[NB] = total;
   }
   wave_sync();
+  // bucket start and length of every live slot (reads first, then the permutation writes)
+  int sl[IPT];  // start | length << 16
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    sl[q] = 0;
+    if (kk[q] != kEmpty) {
+      const int b = bi[q] & 0xFFFF;
+      const int s0 = BC[b], s1 = min(BC[b + 1], H2);
+      sl[q] = s0 | (max(s1 - s0, 0) << 16);
+    }
+  }
 #pragma unroll
   for (int q = 0; q < IPT; ++q)
     if (kk[q] != kEmpty) {
-      const int pos = BC[(int)(((uint64_t)(uint32_t)(kk[q] - cmin) * mul) >> 32)] + io[q];
+      const int pos = (sl[q] & 0xFFFF) + (bi[q] >> 16);
       if (pos < H2) perm[pos] = kk[q];
     }
   wave_sync();
+  // in-bucket rank: only buckets holding more than one key need comparisons; position j of
+  // every such bucket is read for all slots at once
 #pragma unroll
-  for (int q = 0; q < IPT; ++q) {
-    rk[q] = -1;
-    if (kk[q] != kEmpty) {
-      const int b = (int)(((uint64_t)(uint32_t)(kk[q] - cmin) * mul) >> 32);
-      const int s0 = BC[b], s1 = min(BC[b + 1], H2);
-      int r = s0;
-      for (int m = s0; m < s1; ++m) r += (perm[m] < kk[q]);
-      rk[q] = r;
+  for (int q = 0; q < IPT; ++q) rk[q] = (kk[q] != kEmpty) ? (sl[q] & 0xFFFF) : -1;
+  for (int j = 0;; ++j) {
+    bool more = false;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const int len = sl[q] >> 16;
+      if (len > 1 && j < len) rk[q] += (perm[(sl[q] & 0xFFFF) + j] < kk[q]);
+      more |= (len > 1 && j + 1 < len);
     }
+    if (!__builtin_amdgcn_ballot_w64(more)) break;
   }
   return total;
 }
@@ -1007,6 +1039,7 @@ __device__ __forceinline__ void numeric_hash_body(
     rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, Cptr, t, bstart, blen, Aval);
     Step<4, V> st0;  // first gather step of the current row, issued during the previous row
     gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx, Bval);
+    int flags = 0;   // error bits of all rows of this wave, reported once after the loop
     for (;;) {
       rp.prefetch(t, rows, Aptr, Cptr, bstart, blen, Aval);
 #pragma unroll
@@ -1017,35 +1050,50 @@ __device__ __forceinline__ void numeric_hash_body(
       wave_sync();
       HashSink<T, V, true> sk{K, H};
       for_row_products_pre<64, G, 4, true, V>(t, rp.a0, rp.a1, rp.d, bstart, blen, Aval, Bidx, Bval, sk, &st0);
+      // the next row's first gathers (past the last row: the current row again, unused); they
+      // are in flight while this row is ranked and stored
+      gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx, Bval);
       wave_sync();
       const int cmin = wave_reduce_dpp(sk.cmin, OpMin());
       const int cmax = wave_reduce_dpp(sk.cmax, OpMax());
-      int bad = sk.bad;
       int kk[IPT], rk[IPT];
       const int n = wave_rank_table<T>(t, K, cmin, cmax, kk, rk);
-      bad |= (n > T / 2);
-      // the next row's first gathers (past the last row: the current row again, unused)
-      gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx, Bval);
-      // each live slot goes straight to its output position (the row's output is a few KiB;
-      // the partial lines of neighbouring lanes merge in L2)
       const int64_t o0 = rp.c0 - cbase;
       const int64_t nexp = rp.c1 - rp.c0;
+      flags |= (sk.bad || n > T / 2) ? kErrTableFull : 0;
+      flags |= ((int64_t)n != nexp) ? kErrCountMismatch : 0;
+      // stage the ranked row in the (now free) key table: key at its rank, source slot at
+      // T/2 + rank; the values stay in their hash slots
+      wave_sync();
 #pragma unroll
-      for (int q = 0; q < IPT; ++q) {
-        const V vq = H[IPT * t + q];
-        if (rk[q] >= 0 && rk[q] < nexp) {
-          Cidx[o0 + rk[q]] = kk[q];
-          Cval[o0 + rk[q]] = vq;
+      for (int q = 0; q < IPT; ++q)
+        if (rk[q] >= 0 && rk[q] < T / 2) {
+          K[rk[q]] = kk[q];
+          K[T / 2 + rk[q]] = IPT * t + q;
         }
-      }
-      bad = wave_reduce_dpp(bad, OpMax());
-      if (t == 0) {
-        if (bad) atomicOr(err, kErrTableFull);
-        if ((int64_t)n != nexp) atomicOr(err, kErrCountMismatch);
+      wave_sync();
+      // a fixed number of unconditional, coalesced stores: lanes past the row's end repeat its
+      // last element (same address, same data). Unconditional stores keep the compiler's count
+      // of outstanding memory operations exact, so waiting for the next row's gathers never
+      // waits for these stores. A row that came out empty (only after an inconsistency, which
+      // is reported) writes to the sink behind the error word instead of the output.
+      const int nw = (int)min((int64_t)min(n, T / 2), nexp);
+      int32_t* di = nw > 0 ? Cidx + o0 : (int32_t*)(err + 4);
+      V* dv = nw > 0 ? Cval + o0 : (V*)(err + 8);
+      const int last = max(nw, 1) - 1;
+#pragma unroll
+      for (int j = 0; j < (T / 2 + 63) / 64; ++j) {
+        const int m = min(t + 64 * j, last);
+        const int key = K[m];
+        const V val = H[K[T / 2 + m] & (T - 1)];
+        di[m] = key;
+        dv[m] = val;
       }
       wave_sync();
       if (!rp.advance()) break;
     }
+    flags = wave_reduce_dpp(flags, OpOr());
+    if (t == 0 && flags) atomicOr(err, flags);
     return;
   } else {
     int* BC = bc[g];
@@ -1149,7 +1197,7 @@ __device__ __forceinline__ void numeric_hash_body(
 // Occupancy target of the one-wave-per-row numeric kernel (waves per SIMD); the compiler fits
 // its registers to it. The workgroup-per-row shapes are limited by LDS instead.
 #ifndef DBD_NUMERIC_WAVES_PER_EU
-#define DBD_NUMERIC_WAVES_PER_EU 4
+#define DBD_NUMERIC_WAVES_PER_EU 5
 #endif
 
 #define DBD_NUMERIC_ARGS                                                                                     \
@@ -1335,6 +1383,7 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
           const int lv = __shfl(d.len, src);  // on all lanes, then masked
           const int pl = e < cnt ? lv : 0;
           const int64_t pb = __shfl(d.b0, src);
+          const V pva = __shfl(d.va, src);
           acc[u] = V(0);
           if (lg < pl) {
             const int s = hash_find<T>(K, st.c[u]);
@@ -1343,7 +1392,7 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
             } else {
               const V gv = H[s];
               acc[u] = gv * st.bv[u];
-              if (dB) atomicAdd(&dB[pb + lg], st.pv[u] * gv);
+              if (dB) atomicAdd(&dB[pb + lg], pva * gv);
             }
           }
           for (int q = G + lg; q < pl; q += G) {
@@ -1355,7 +1404,7 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
             }
             const V gv = H[s];
             acc[u] += gv * Bval[p];
-            if (dB) atomicAdd(&dB[p], st.pv[u] * gv);
+            if (dB) atomicAdd(&dB[p], pva * gv);
           }
         }
 #pragma unroll
@@ -1472,12 +1521,30 @@ static int device_cus() {
   return cus[dev];
 }
 
-static int grid_for(int64_t nrows, int gpb, int grid) {
+// Workgroups of kernel f (block threads) resident on one CU, from the occupancy calculator.
+static int resident_blocks(const void* f, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_pair(f, block * 64 + dev);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 4;
+  cache[key] = n;
+  return n;
+}
+
+// Grid of a row kernel: `grid` if given (persistent launches), else one group per row. The
+// one-wave-per-row kernels (gpb > 1) walk many rows per wave, their row pipeline amortising
+// over them: their grid is capped at what is resident at once, so no second, partly filled
+// round of workgroups follows the first (all workgroups get equal row counts).
+static int grid_for(int64_t nrows, int gpb, int grid, const void* f = nullptr, int block = 256) {
   if (grid > 0) return grid;
   int64_t g = (nrows + gpb - 1) / gpb;
-  // one-wave-per-row kernels walk many rows per wave (their row pipeline amortises over the
-  // rows): twice the workgroups that are resident at 4 per CU, a multiple of the 8 XCDs
-  if (gpb > 1) g = std::min<int64_t>(g, (int64_t)device_cus() * 8);
+  if (gpb > 1 && f) g = std::min<int64_t>(g, (int64_t)device_cus() * resident_blocks(f, block));
   if (g > 1048576) g = 1048576;
   if (g < 1) g = 1;
   return (int)g;
@@ -1510,14 +1577,15 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
                      int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
                      int* err, hipStream_t s) {
   const int gpb = symbolic_groups_per_wg(k);
-  const dim3 gr(grid_for(nrows_hint, gpb, grid));
   if (k.T == 0) {
+    const dim3 gr(grid_for(nrows_hint, gpb, grid));
     hipLaunchKernelGGL((k_symbolic_long<1024, kSymLongW, kLongG>), gr, dim3(1024), 0, s, rows, pbeg, pend, A.ptr,
                        A.idx, B.ptr, B.idx, nnz, cursor, err);
     return;
   }
 #define X(T_, NT_, G_)                                                                                         \
   if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                               \
+    const dim3 gr(grid_for(nrows_hint, gpb, grid, (const void*)k_symbolic_hash<T_, NT_, G_>, wg_of<NT_>())); \
     hipLaunchKernelGGL((k_symbolic_hash<T_, NT_, G_>), gr, dim3(wg_of<NT_>()), 0, s, rows, pbeg, pend, A.ptr, \
                        A.bstart, A.blen, B.idx, nnz, err);                                                   \
     return;                                                                                                   \
@@ -1534,10 +1602,10 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
                              int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr,
                              int64_t cbase, int32_t* Cidx, V* Cval, int64_t* cursor, int* err, hipStream_t s) {
   const int gpb = symbolic_groups_per_wg(k);
-  const dim3 gr(grid_for(nrows_hint, gpb, grid));
   const V* Av = (const V*)A.val;
   const V* Bv = (const V*)B.val;
   if (k.T == 0) {
+    const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int W = sizeof(V) == 8 ? kNumLongW64 : kNumLongW32;
     hipLaunchKernelGGL((k_numeric_long<V, kLongNT, W, kLongG>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
                        A.idx, Av, B.ptr, B.idx, Bv, Cptr, cbase, Cidx, Cval, cursor, err);
@@ -1545,12 +1613,15 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
   }
 #define X(T_, NT_, G_)                                                                                           \
   if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                                 \
-    if constexpr (NT_ == 64)                                                                                   \
+    if constexpr (NT_ == 64) {                                                                                 \
+      const dim3 gr(grid_for(nrows_hint, gpb, grid, (const void*)k_numeric_wave<V, T_, NT_, G_>, 256));        \
       hipLaunchKernelGGL((k_numeric_wave<V, T_, NT_, G_>), gr, dim3(256), 0, s, rows, pbeg, pend, A.ptr,       \
                          A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                      \
-    else                                                                                                        \
+    } else {                                                                                                    \
+      const dim3 gr(grid_for(nrows_hint, gpb, grid));                                                          \
       hipLaunchKernelGGL((k_numeric_block<V, T_, NT_, G_>), gr, dim3(NT_), 0, s, rows, pbeg, pend, A.ptr,      \
                          A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                      \
+    }                                                                                                           \
     return;                                                                                                     \
   }
   DBD_SHAPES(X, 16)
@@ -1575,10 +1646,10 @@ static void launch_backward_t(const KernelShape& k, const int32_t* rows, const i
                               const int32_t* Cidx, const V* dC, V* dA, V* dB, int64_t* cursor, int* err,
                               hipStream_t s) {
   const int gpb = symbolic_groups_per_wg(k);
-  const dim3 gr(grid_for(nrows_hint, gpb, grid));
   const V* Av = (const V*)A.val;
   const V* Bv = (const V*)B.val;
   if (k.T == 0) {
+    const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int W = sizeof(V) == 8 ? kBwdLongW64 : kBwdLongW32;
     hipLaunchKernelGGL((k_backward_long<V, kLongNT, W, kLongG>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
                        A.idx, Av, B.ptr, B.idx, Bv, Cptr, Cidx, dC, dA, dB, cursor, err);
@@ -1586,6 +1657,7 @@ static void launch_backward_t(const KernelShape& k, const int32_t* rows, const i
   }
 #define X(T_, NT_, G_)                                                                                             \
   if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                                   \
+    const dim3 gr(grid_for(nrows_hint, gpb, grid, (const void*)k_backward_hash<V, T_, NT_, G_>, wg_of<NT_>())); \
     hipLaunchKernelGGL((k_backward_hash<V, T_, NT_, G_>), gr, dim3(wg_of<NT_>()), 0, s, rows, pbeg, pend, A.ptr, \
                        A.bstart, A.blen, Av, B.idx, Bv, Cptr, Cidx, dC, dA, dB, err);                            \
     return;                                                                                                       \
