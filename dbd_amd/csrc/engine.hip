@@ -429,7 +429,8 @@ dbd_status plan(dbd_ctx* c) {
   return DBD_OK;
 }
 
-dbd_status ensure_cursor(dbd_ctx* c) { return c->cursor.ensure(8 * (size_t)(c->A.nnz + 1)); }
+// window-kernel workspace: 16 bytes per A nonzero (cursor state of long rows)
+dbd_status ensure_cursor(dbd_ctx* c) { return c->cursor.ensure(16 * (size_t)(c->A.nnz + 1)); }
 
 // Symbolic pass over all rows -> nnz_i -> Cptr (device, ctx-owned) -> numeric bins.
 dbd_status symbolic(dbd_ctx* c) {

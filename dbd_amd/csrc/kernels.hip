@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <utility>
 #include <mutex>
 #include <map>
 
@@ -285,6 +286,33 @@ __device__ __forceinline__ int hash_find(const int* hk, int c) {
   return -1;
 }
 
+// Slots a lane owns in a one-wave row table of T slots (T/64 per lane): 16-byte chunks
+// t, t + 64, ... (fewer slots per lane: the slots t*IPT ..), so a wide LDS access of the whole
+// wave covers one contiguous block and hits every bank once.
+template <int T>
+__device__ __forceinline__ int slot_of(int t, int q) {
+  constexpr int IPT = T / 64;
+  constexpr int CW = IPT < 4 ? IPT : 4;
+  return CW * ((q / CW) * 64 + t) + q % CW;
+}
+
+// Clear a one-wave row table (keys to kEmpty, values to 0) with contiguous 16-byte stores.
+template <int T, typename V>
+__device__ __forceinline__ void clear_table(int t, int* K, V* H) {
+  if constexpr (T >= 256) {
+#pragma unroll
+    for (int c = 0; c < T / 256; ++c) reinterpret_cast<int4*>(K)[c * 64 + t] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+  } else {
+#pragma unroll
+    for (int q = 0; q < T / 64; ++q) K[q * 64 + t] = kEmpty;
+  }
+  if (H) {
+    constexpr int CH = T * (int)sizeof(V) / 16;  // 16-byte chunks of values
+#pragma unroll
+    for (int c = t; c < CH; c += 64) reinterpret_cast<int4*>(H)[c] = make_int4(0, 0, 0, 0);
+  }
+}
+
 // Descriptor of the A entry a lane holds: start and length of the selected B row, A's value.
 template <typename V>
 struct ADesc {
@@ -293,16 +321,63 @@ struct ADesc {
   V va;
 };
 
-// Lane `lane` loads the descriptor of A entry base + lane. These loads are sequential in A's
-// order (coalesced). Lanes past a1 load entry a1 - 1 instead, and the values are kept raw: the
-// consumers (gather_step, consume_step) take only entries below the chunk's count. Unconditional loads keep
-// the compiler's in-order count of outstanding loads exact (a branch around a load, or a select
-// on its result right after it, makes the wave wait for the load at that point). a1 >= 1 for
-// every row that reaches a kernel (such rows have products).
-template <bool WITHV, typename V>
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time (for indices
+// that must be immediates, such as DPP lane selects).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Which A entry of a 64-entry chunk a lane holds. With G = 16 the layout is transposed: lane
+// 16r + j holds entry 4j + r, so in step S (entries 16S .. 16S+15) subgroup r's u-th entry,
+// 16S + 4u + r, sits in lane 4S + u of the subgroup's own row of 16 lanes and reaches all of
+// the row through one DPP row broadcast (no LDS traffic). With G = 64 lane e holds entry e and
+// the wave takes it with readlane.
+template <int G>
+__device__ __forceinline__ int desc_slot(int lane) {
+  static_assert(G == 16 || G == 64, "walk widths");
+  return G == 16 ? 4 * (lane & 15) + (lane >> 4) : lane;
+}
+
+// Field of the entry at broadcast index J (G = 16: lane J of each row; G = 64: lane e) to all
+// lanes that walk it.
+template <int G, int J>
+__device__ __forceinline__ int ebcast(int x, int e) {
+  if constexpr (G == 16)
+    return __builtin_amdgcn_update_dpp(0, x, 0x150 + J, 0xF, 0xF, false);  // row_newbcast:J
+  else
+    return __builtin_amdgcn_readlane(x, e);
+}
+template <int G, int J>
+__device__ __forceinline__ int64_t ebcast64(int64_t x, int e) {
+  const uint32_t lo = (uint32_t)ebcast<G, J>((int)(uint32_t)x, e);
+  const uint32_t hi = (uint32_t)ebcast<G, J>((int)(uint32_t)((uint64_t)x >> 32), e);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <int G, int J>
+__device__ __forceinline__ double ebcastv(double x, int e) {
+  return __longlong_as_double(ebcast64<G, J>(__double_as_longlong(x), e));
+}
+template <int G, int J>
+__device__ __forceinline__ float ebcastv(float x, int e) {
+  return __int_as_float(ebcast<G, J>(__float_as_int(x), e));
+}
+
+// Lane `lane` loads the descriptor of A entry base + desc_slot(lane). These loads cover the
+// chunk's entries in A's order (coalesced). Lanes past a1 load entry a1 - 1 instead, and the
+// values are kept raw: the consumers (gather_step, consume_step) take only entries below the
+// chunk's count. Unconditional loads keep the compiler's in-order count of outstanding loads
+// exact (a branch around a load, or a select on its result right after it, makes the wave wait
+// for the load at that point). a1 >= 1 for every row that reaches a kernel (such rows have
+// products).
+template <int G, bool WITHV, typename V>
 __device__ __forceinline__ ADesc<V> load_desc(int64_t base, int64_t a1, int lane, const int64_t* __restrict__ bstart,
                                               const int32_t* __restrict__ blen, const V* __restrict__ Aval) {
-  const int64_t j = base + lane;
+  const int64_t j = base + desc_slot<G>(lane);
   const int64_t jj = j < a1 ? j : (a1 > 0 ? a1 - 1 : 0);
   ADesc<V> d;
   d.b0 = bstart[jj];
@@ -311,85 +386,99 @@ __device__ __forceinline__ ADesc<V> load_desc(int64_t base, int64_t a1, int lane
   return d;
 }
 
-// One gather step over a chunk of A entries whose descriptors are spread over the wave's
-// lanes: the wave's 64/G subgroups take entries through wave shuffles, U entries per step, and
-// issue the U B-row gathers (the first G entries of each B row). Only the loaded columns and B
-// values are kept; they are masked and multiplied when the step is consumed.
+// One gather step over a chunk of A entries: the wave's 64/G subgroups take U entries each
+// (G = 16: step S, entries 16S .. 16S+15; G = 64: entries e0 .. e0+U-1) and issue the U B-row
+// gathers (the first G entries of each B row). Only the loaded columns and B values are kept;
+// they are masked and multiplied when the step is consumed.
 template <int U, typename V>
 struct Step {
   int c[U];   // B columns as loaded (masked when consumed)
   V bv[U];    // B values as loaded
 };
 
-template <int G, int U, bool WITHV, typename V>
+// index of entry u of this lane's subgroup in the step
+template <int G, int S>
+__device__ __forceinline__ int step_entry(int e0, int u, int sg) {
+  return G == 16 ? 16 * S + 4 * u + sg : e0 + u;
+}
+
+template <int G, int U, bool WITHV, typename V, int S = 0>
 __device__ __forceinline__ void gather_step(Step<U, V>& st, int lane, int e0, int cnt, const ADesc<V>& d,
                                             const int32_t* __restrict__ Bidx, const V* __restrict__ Bval) {
-  static_assert(64 % G == 0, "subgroups tile the wave");
-  constexpr int NG = 64 / G;
+  static_assert(G == 64 || U == 4, "with G = 16 a step is 4 subgroups x 4 entries");
   const int sg = lane / G, lg = lane % G;
   // unconditional loads: lanes past the B row re-read its last entry, entries without a B row
   // read entry 0 (B has entries whenever a row reaches a kernel). The loaded values are kept
   // raw: masking them here would make the wave wait for the loads right away.
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int e = e0 + u * NG + sg;
-    const int src = e < 64 ? e : 63;
-    // shuffles run on all lanes (a lane reading from an inactive lane gets 0), then mask
-    const int64_t pb = __shfl(d.b0, src);
-    const int lv = __shfl(d.len, src);
+  static_for<U>([&](auto uc) {
+    constexpr int u = decltype(uc)::value;
+    const int e = step_entry<G, S>(e0, u, sg);
+    const int er = min(e0 + u, 63);
+    const int64_t pb = ebcast64<G, 4 * S + (u & 3)>(d.b0, er);
+    const int lv = ebcast<G, 4 * S + (u & 3)>(d.len, er);
     const int pl = e < cnt ? lv : 0;
     const int64_t at = pl > 0 ? pb + min(lg, pl - 1) : 0;
     st.c[u] = Bidx[at];
     st.bv[u] = WITHV ? Bval[at] : V(0);
-  }
+  });
 }
 
 // Hand a gathered step to the sink: f.batch(c[U], v[U]) for the first G entries of each B row
 // (c < 0 marks no product), f.one(c, v) for the tails of B rows longer than G; v = A_ik * B_kj.
-template <int G, int U, bool WITHV, typename V, typename F>
+template <int G, int U, bool WITHV, typename V, typename F, int S = 0>
 __device__ __forceinline__ void consume_step(const Step<U, V>& st, int lane, int e0, int cnt, const ADesc<V>& d,
                                              const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F& f) {
-  constexpr int NG = 64 / G;
   const int sg = lane / G, lg = lane % G;
   int pl[U], c[U];
   V v[U], pv[U];
   bool tail = false;
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int e = e0 + u * NG + sg;
-    const int lv = __shfl(d.len, e < 64 ? e : 63);  // on all lanes, then masked
-    pv[u] = WITHV ? __shfl(d.va, e < 64 ? e : 63) : V(0);
+  static_for<U>([&](auto uc) {
+    constexpr int u = decltype(uc)::value;
+    const int e = step_entry<G, S>(e0, u, sg);
+    const int er = min(e0 + u, 63);
+    const int lv = ebcast<G, 4 * S + (u & 3)>(d.len, er);
+    if constexpr (WITHV) pv[u] = ebcastv<G, 4 * S + (u & 3)>(d.va, er);
+    else pv[u] = V(0);
     pl[u] = e < cnt ? lv : 0;
     const bool has = lg < pl[u];
     c[u] = has ? st.c[u] : -1;
     v[u] = (WITHV && has) ? pv[u] * st.bv[u] : V(0);
     tail |= pl[u] > G;
-  }
+  });
   f.template batch<U>(c, v);
   if (__builtin_amdgcn_ballot_w64(tail)) {  // B rows longer than G: the rest of them, one by one
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = e0 + u * NG + sg;
-      const int64_t pb = __shfl(d.b0, e < 64 ? e : 63);
+    static_for<U>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      const int64_t pb = ebcast64<G, 4 * S + (u & 3)>(d.b0, min(e0 + u, 63));
       for (int q = G + lg; q < pl[u]; q += G) {
         const int64_t p = pb + q;
         f.one(Bidx[p], WITHV ? pv[u] * Bval[p] : V(0));
       }
-    }
+    });
     wait_vmem();
   }
 }
 
-// All products of `cnt` (<= 64) A entries of one chunk, steps of U entries per subgroup.
-// Control flow around the shuffles is wave-uniform. `from` skips steps already consumed.
+template <int G, int U, bool WITHV, typename V, typename F, int S>
+__device__ __forceinline__ void walk_step(int lane, int e0, int cnt, const ADesc<V>& d,
+                                          const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F& f) {
+  Step<U, V> st;
+  gather_step<G, U, WITHV, V, S>(st, lane, e0, cnt, d, Bidx, Bval);
+  consume_step<G, U, WITHV, V, F, S>(st, lane, e0, cnt, d, Bidx, Bval, f);
+}
+
+// All products of `cnt` (<= 64) A entries of one chunk, from entry `from` on (a multiple of a
+// step). Control flow around the broadcasts is wave-uniform.
 template <int G, int U, bool WITHV, typename V, typename F>
 __device__ __forceinline__ void walk_chunk(int lane, int cnt, const ADesc<V>& d, const int32_t* __restrict__ Bidx,
                                            const V* __restrict__ Bval, F& f, int from = 0) {
-  constexpr int NG = 64 / G;
-  for (int e0 = from; e0 < cnt; e0 += NG * U) {
-    Step<U, V> st;
-    gather_step<G, U, WITHV, V>(st, lane, e0, cnt, d, Bidx, Bval);
-    consume_step<G, U, WITHV, V>(st, lane, e0, cnt, d, Bidx, Bval, f);
+  if constexpr (G == 16) {  // steps are compile-time (their broadcast lanes are DPP immediates)
+    if (from <= 0 && 0 < cnt) walk_step<G, U, WITHV, V, F, 0>(lane, 0, cnt, d, Bidx, Bval, f);
+    if (from <= 16 && 16 < cnt) walk_step<G, U, WITHV, V, F, 1>(lane, 16, cnt, d, Bidx, Bval, f);
+    if (from <= 32 && 32 < cnt) walk_step<G, U, WITHV, V, F, 2>(lane, 32, cnt, d, Bidx, Bval, f);
+    if (from <= 48 && 48 < cnt) walk_step<G, U, WITHV, V, F, 3>(lane, 48, cnt, d, Bidx, Bval, f);
+  } else {
+    for (int e0 = from; e0 < cnt; e0 += U) walk_step<G, U, WITHV, V, F, 0>(lane, e0, cnt, d, Bidx, Bval, f);
   }
 }
 
@@ -403,15 +492,15 @@ __device__ __forceinline__ void for_row_products_pre(int t, int64_t a0, int64_t 
                                                      const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
                                                      F& f, const Step<U, V>* step0 = nullptr) {
   static_assert(NT >= 64, "wave-level walk");
-  constexpr int NG = 64 / G;
+  constexpr int STEP = G == 16 ? 16 : U;  // entries per step
   const int lane = t & 63, wv = t >> 6;
   int64_t base = a0 + 64 * wv;
   if (base < a1) {
     const int cnt = (int)min((int64_t)64, a1 - base);
     int from = 0;
     if (step0) {  // the caller already issued the first step's gathers
-      consume_step<G, U, WITHV, V>(*step0, lane, 0, cnt, first, Bidx, Bval, f);
-      from = NG * U;
+      consume_step<G, U, WITHV, V, F, 0>(*step0, lane, 0, cnt, first, Bidx, Bval, f);
+      from = STEP;
     }
     if (from < cnt) {
       walk_chunk<G, U, WITHV, V>(lane, cnt, first, Bidx, Bval, f, from);
@@ -420,7 +509,7 @@ __device__ __forceinline__ void for_row_products_pre(int t, int64_t a0, int64_t 
   }
   if (base + NT < a1) {
     for (base += NT; base < a1; base += NT) {
-      const ADesc<V> d = load_desc<WITHV, V>(base, a1, lane, bstart, blen, Aval);
+      const ADesc<V> d = load_desc<G, WITHV, V>(base, a1, lane, bstart, blen, Aval);
       walk_chunk<G, U, WITHV, V>(lane, (int)min((int64_t)64, a1 - base), d, Bidx, Bval, f);
     }
     if (step0) wait_vmem();
@@ -431,7 +520,7 @@ template <int NT, int G, int U, bool WITHV, typename V, typename F>
 __device__ __forceinline__ void for_row_products(int t, int64_t a0, int64_t a1, const int64_t* __restrict__ bstart,
                                                  const int32_t* __restrict__ blen, const V* __restrict__ Aval,
                                                  const int32_t* __restrict__ Bidx, const V* __restrict__ Bval, F& f) {
-  const ADesc<V> first = load_desc<WITHV, V>(a0 + 64 * (t >> 6), a1, t & 63, bstart, blen, Aval);
+  const ADesc<V> first = load_desc<G, WITHV, V>(a0 + 64 * (t >> 6), a1, t & 63, bstart, blen, Aval);
   for_row_products_pre<NT, G, U, WITHV, V>(t, a0, a1, first, bstart, blen, Aval, Bidx, Bval, f);
 }
 
@@ -450,7 +539,7 @@ __device__ __forceinline__ int64_t xcd_block() {
 // while the wave waits on LDS (scalar loads and LDS share one counter). The next row's A
 // descriptors are issued at the top of each iteration and its first B gathers after the
 // current row's products are in LDS, so each wait is for loads issued a good while earlier.
-template <bool WITHV, typename V>
+template <int G, bool WITHV, typename V>
 struct RowPipe {
   int64_t slot, stride, end;   // slot of the current row (wave-uniform)
   int kb;                      // batch index of the next row (wave-uniform)
@@ -494,7 +583,7 @@ struct RowPipe {
     load_batch(s0, lane, rows, Aptr, Cptr);
     kb = 0;
     extract(0, row, a0, a1, c0, c1);
-    d = load_desc<WITHV, V>(a0, a1, lane, bstart, blen, Aval);
+    d = load_desc<G, WITHV, V>(a0, a1, lane, bstart, blen, Aval);
   }
   // top of an iteration: pointers (a new batch every 64 rows) and descriptors of the next row;
   // past the last row the current row is repeated, so the loads stay unconditional
@@ -512,7 +601,7 @@ struct RowPipe {
       }
     }
     extract(kb, row1, na0, na1, nc0, nc1);
-    nd = load_desc<WITHV, V>(na0, na1, lane, bstart, blen, Aval);
+    nd = load_desc<G, WITHV, V>(na0, na1, lane, bstart, blen, Aval);
   }
   __device__ __forceinline__ bool advance() {
     if (!more) return false;
@@ -779,7 +868,7 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
   if constexpr (NT == 64) {
     const int64_t s0 = beg + xcd_block() * GPB + g;
     if (s0 >= end) return;
-    RowPipe<false, float> rp;
+    RowPipe<G, false, float> rp;
     rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, nullptr, t, bstart, blen, (const float*)nullptr);
     Step<4, float> st0;  // first gather step of the current row, issued during the previous row
     gather_step<G, 4, false, float>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx,
@@ -789,8 +878,7 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
     int crow = 0, ccnt = 0, jn = 0, flags = 0;
     for (;;) {
       rp.prefetch(t, rows, Aptr, nullptr, bstart, blen, (const float*)nullptr);
-#pragma unroll
-      for (int q = 0; q < T / 64; ++q) K[(T / 64) * t + q] = kEmpty;
+      clear_table<T, float>(t, K, nullptr);
       wave_sync();
       HashSink<T, float, false, false> sk{K, nullptr};
       for_row_products_pre<64, G, 4, false, float>(t, rp.a0, rp.a1, rp.d, bstart, blen, (const float*)nullptr, Bidx,
@@ -918,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 // fits the buckets) is monotone, so a key's rank is the start of its bucket plus the number of
 // smaller keys in the bucket; mul only has to keep b below NB, so it is formed in single
 // precision and rounded down. LDS reads of all slots are issued before their results are used.
-// rk[q] = rank of slot IPT*t + q, -1 if empty. Returns the number of live entries (more than
+// rk[q] = rank of slot slot_of(t, q), -1 if empty. Returns the number of live entries (more than
 // T/2 means the table was overfilled; ranks are then not exact and the caller reports it).
 template <int T>
 __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax, int (&kk)[T / 64],
@@ -935,7 +1023,7 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
   int* BC = K;
   int* perm = K + H2;
 #pragma unroll
-  for (int q = 0; q < IPT; ++q) kk[q] = K[IPT * t + q];
+  for (int q = 0; q < IPT; ++q) kk[q] = K[slot_of<T>(t, q)];
   wave_sync();
 #pragma unroll
   for (int q = 0; q < PER; ++q)
@@ -1035,18 +1123,14 @@ __device__ __forceinline__ void numeric_hash_body(
     // wave_rank_table while the next row's descriptors are in flight
     const int64_t s0 = beg + xcd_block() * GPB + g;
     if (s0 >= end) return;
-    RowPipe<true, V> rp;
+    RowPipe<G, true, V> rp;
     rp.start(s0, (int64_t)gridDim.x * GPB, end, rows, Aptr, Cptr, t, bstart, blen, Aval);
     Step<4, V> st0;  // first gather step of the current row, issued during the previous row
     gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx, Bval);
     int flags = 0;   // error bits of all rows of this wave, reported once after the loop
     for (;;) {
       rp.prefetch(t, rows, Aptr, Cptr, bstart, blen, Aval);
-#pragma unroll
-      for (int q = 0; q < IPT; ++q) {  // lane t owns slots [IPT*t, IPT*t + IPT): wide stores
-        K[IPT * t + q] = kEmpty;
-        H[IPT * t + q] = V(0);
-      }
+      clear_table<T, V>(t, K, H);
       wave_sync();
       HashSink<T, V, true> sk{K, H};
       for_row_products_pre<64, G, 4, true, V>(t, rp.a0, rp.a1, rp.d, bstart, blen, Aval, Bidx, Bval, sk, &st0);
@@ -1069,7 +1153,7 @@ __device__ __forceinline__ void numeric_hash_body(
       for (int q = 0; q < IPT; ++q)
         if (rk[q] >= 0 && rk[q] < T / 2) {
           K[rk[q]] = kk[q];
-          K[T / 2 + rk[q]] = IPT * t + q;
+          K[T / 2 + rk[q]] = slot_of<T>(t, q);
         }
       wave_sync();
       // a fixed number of unconditional, coalesced stores: lanes past the row's end repeat its
@@ -1324,6 +1408,57 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
 // ------------------------------------------------------------------------------------
 // backward: dA_ik = sum_j dC_ij B_kj ; dB_kj += A_ik dC_ij  (pattern of C from forward)
 // ------------------------------------------------------------------------------------
+// One step of the backward walk: U entries per subgroup (as in gather_step); every product
+// looks its column up in the row's dC table, dA_ik is reduced over the subgroup and
+// A_ik * dC_ij is added into dB.
+template <int T, int G, int U, typename V, int S>
+__device__ __forceinline__ void bwd_step(int lane, int e0, int cnt, int64_t base, const ADesc<V>& d, const int* K,
+                                         const V* H, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+                                         V* __restrict__ dA, V* __restrict__ dB, int& bad) {
+  const int sg = lane / G, lg = lane % G;
+  Step<U, V> st;
+  gather_step<G, U, true, V, S>(st, lane, e0, cnt, d, Bidx, Bval);
+  V acc[U];
+  static_for<U>([&](auto uc) {
+    constexpr int u = decltype(uc)::value;
+    const int e = step_entry<G, S>(e0, u, sg);
+    const int er = min(e0 + u, 63);
+    const int lv = ebcast<G, 4 * S + (u & 3)>(d.len, er);
+    const int pl = e < cnt ? lv : 0;
+    const int64_t pb = ebcast64<G, 4 * S + (u & 3)>(d.b0, er);
+    const V pva = ebcastv<G, 4 * S + (u & 3)>(d.va, er);
+    acc[u] = V(0);
+    if (lg < pl) {
+      const int s = hash_find<T>(K, st.c[u]);
+      if (s < 0) {
+        bad = 1;  // product column absent from C's row: C does not belong to these operands
+      } else {
+        const V gv = H[s];
+        acc[u] = gv * st.bv[u];
+        if (dB) atomicAdd(&dB[pb + lg], pva * gv);
+      }
+    }
+    for (int q = G + lg; q < pl; q += G) {
+      const int64_t p = pb + q;
+      const int s = hash_find<T>(K, Bidx[p]);
+      if (s < 0) {
+        bad = 1;
+        continue;
+      }
+      const V gv = H[s];
+      acc[u] += gv * Bval[p];
+      if (dB) atomicAdd(&dB[p], pva * gv);
+    }
+  });
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o, G);
+    const int e = step_entry<G, S>(e0, u, sg);
+    if (dA && lg == 0 && e < cnt) dA[base + e] = acc[u];
+  }
+}
+
 // Per C row: dC_i goes into an LDS table keyed by column; then every A entry of the row walks
 // its B row (descriptor chunks, U gathers in flight per subgroup as in the forward kernels),
 // looks each product column up, reduces dA_ik over its subgroup and adds A_ik * dC_ij into dB.
@@ -1371,49 +1506,14 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
     for (int64_t base = a0 + 64 * wv; base < a1; base += NT) {  // wave-uniform
       const int cnt = (int)min((int64_t)64, a1 - base);
-      const ADesc<V> d = load_desc<true, V>(base, a1, lane, bstart, blen, Aval);
-      for (int e0 = 0; e0 < cnt; e0 += NG * U) {
-        Step<U, V> st;
-        gather_step<G, U, true, V>(st, lane, e0, cnt, d, Bidx, Bval);
-        V acc[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int e = e0 + u * NG + sg;
-          const int src = e < 64 ? e : 63;
-          const int lv = __shfl(d.len, src);  // on all lanes, then masked
-          const int pl = e < cnt ? lv : 0;
-          const int64_t pb = __shfl(d.b0, src);
-          const V pva = __shfl(d.va, src);
-          acc[u] = V(0);
-          if (lg < pl) {
-            const int s = hash_find<T>(K, st.c[u]);
-            if (s < 0) {
-              bad = 1;  // product column absent from C's row: C does not belong to these operands
-            } else {
-              const V gv = H[s];
-              acc[u] = gv * st.bv[u];
-              if (dB) atomicAdd(&dB[pb + lg], pva * gv);
-            }
-          }
-          for (int q = G + lg; q < pl; q += G) {
-            const int64_t p = pb + q;
-            const int s = hash_find<T>(K, Bidx[p]);
-            if (s < 0) {
-              bad = 1;
-              continue;
-            }
-            const V gv = H[s];
-            acc[u] += gv * Bval[p];
-            if (dB) atomicAdd(&dB[p], pva * gv);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-#pragma unroll
-          for (int o = G / 2; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o, G);
-          const int e = e0 + u * NG + sg;
-          if (dA && lg == 0 && e < cnt) dA[base + e] = acc[u];
-        }
+      const ADesc<V> d = load_desc<G, true, V>(base, a1, lane, bstart, blen, Aval);
+      if constexpr (G == 16) {  // compile-time steps (DPP broadcast lanes are immediates)
+        if (0 < cnt) bwd_step<T, G, U, V, 0>(lane, 0, cnt, base, d, K, H, Bidx, Bval, dA, dB, bad);
+        if (16 < cnt) bwd_step<T, G, U, V, 1>(lane, 16, cnt, base, d, K, H, Bidx, Bval, dA, dB, bad);
+        if (32 < cnt) bwd_step<T, G, U, V, 2>(lane, 32, cnt, base, d, K, H, Bidx, Bval, dA, dB, bad);
+        if (48 < cnt) bwd_step<T, G, U, V, 3>(lane, 48, cnt, base, d, K, H, Bidx, Bval, dA, dB, bad);
+      } else {
+        for (int e0 = 0; e0 < cnt; e0 += U) bwd_step<T, G, U, V, 0>(lane, e0, cnt, base, d, K, H, Bidx, Bval, dA, dB, bad);
       }
     }
     bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
