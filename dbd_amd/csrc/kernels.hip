@@ -1095,6 +1095,92 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
   return total;
 }
 
+// Sort the live entries of a group's row table (T slots, NT > 64 threads, at most T/2 live)
+// by column and write them to Cidx/Cval[0 .. min(n, limit)); returns n. Buckets by the
+// monotone map b(c) = floor((c - cmin) * NB / range), NB = T/2 counters in BC (NB + 1 ints),
+// then the rank inside a bucket by comparison. The table's key and value LDS are reused.
+// Leaves the table (K, H) unspecified and BC dirty; the caller clears them.
+template <typename V, int T, int NT>
+__device__ int block_rank_emit(int t, int* K, V* H, int* BC, int* sc, int cmin, int cmax, int32_t* __restrict__ Cidx,
+                               V* __restrict__ Cval, int64_t limit) {
+  constexpr int IPT = T / NT;
+  constexpr int NB = T / 2;
+  constexpr int PER = (NB + NT - 1) / NT;
+  for (int s2 = t; s2 <= NB; s2 += NT) BC[s2] = 0;
+  gsync<NT>();
+  const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
+  const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
+  int ik[IPT], ib[IPT], io[IPT];
+  V iv[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int s2 = t + j * NT;
+    const int key = K[s2];
+    ik[j] = key;
+    iv[j] = H[s2];
+    ib[j] = 0;
+    io[j] = 0;
+    if (key != kEmpty) {
+      const int b = (int)(((uint64_t)(uint32_t)(key - cmin) * mul) >> 32);
+      ib[j] = b;
+      io[j] = atomicAdd(&BC[b], 1);
+    }
+  }
+  gsync<NT>();
+  {  // exclusive scan of bucket counts, PER contiguous buckets per thread
+    int loc[PER];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = t * PER + q;
+      loc[q] = (i < NB) ? BC[i] : 0;
+      sum += loc[q];
+    }
+    int total;
+    int pre = Group<NT>::exscan(sum, total, sc);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = t * PER + q;
+      if (i < NB) BC[i] = pre;
+      pre += loc[q];
+    }
+    if (t == 0) BC[NB] = total;
+  }
+  gsync<NT>();
+  int* perm = K;  // the table keys are in registers now; reuse their LDS
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (ik[j] != kEmpty) perm[BC[ib[j]] + io[j]] = ik[j];
+  gsync<NT>();
+  int rk[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    rk[j] = 0;
+    if (ik[j] != kEmpty) {
+      const int s0 = BC[ib[j]], s1 = BC[ib[j] + 1];
+      int r = s0;
+      for (int m = s0; m < s1; ++m) r += (perm[m] < ik[j]);
+      rk[j] = r;
+    }
+  }
+  gsync<NT>();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j)
+    if (ik[j] != kEmpty) {
+      K[rk[j]] = ik[j];
+      H[rk[j]] = iv[j];
+    }
+  gsync<NT>();
+  const int n = BC[NB];
+  const int nw = (int)min((int64_t)n, limit);
+  for (int m = t; m < nw; m += NT) {
+    Cidx[m] = K[m];
+    Cval[m] = H[m];
+  }
+  gsync<NT>();
+  return n;
+}
+
 template <typename V, int T, int NT, int G>
 __device__ __forceinline__ void numeric_hash_body(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
@@ -1188,7 +1274,6 @@ __device__ __forceinline__ void numeric_hash_body(
         K[s2] = kEmpty;
         H[s2] = V(0);
       }
-      for (int s2 = t; s2 <= NB; s2 += NT) BC[s2] = 0;
       gsync<NT>();
       const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
       HashSink<T, V, true> sk{K, H};
@@ -1197,77 +1282,9 @@ __device__ __forceinline__ void numeric_hash_body(
       const int cmin = Group<NT>::reduce(sk.cmin, OpMin(), sc[g]);
       const int cmax = Group<NT>::reduce(sk.cmax, OpMax(), sc[g]);
       int bad = sk.bad;
-      const uint32_t range = (uint32_t)(cmax - cmin) + 1u;
-      const uint64_t mul = ((uint64_t)NB << 32) / (uint64_t)range;
-      int ik[IPT], ib[IPT], io[IPT];
-      V iv[IPT];
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        const int s2 = t + j * NT;
-        const int key = K[s2];
-        ik[j] = key;
-        iv[j] = H[s2];
-        ib[j] = 0;
-        io[j] = 0;
-        if (key != kEmpty) {
-          const int b = (int)(((uint64_t)(uint32_t)(key - cmin) * mul) >> 32);
-          ib[j] = b;
-          io[j] = atomicAdd(&BC[b], 1);
-        }
-      }
-      gsync<NT>();
-      {  // exclusive scan of bucket counts, PER contiguous buckets per thread
-        int loc[PER];
-        int sum = 0;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          const int i = t * PER + q;
-          loc[q] = (i < NB) ? BC[i] : 0;
-          sum += loc[q];
-        }
-        int total;
-        int pre = Group<NT>::exscan(sum, total, sc[g]);
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          const int i = t * PER + q;
-          if (i < NB) BC[i] = pre;
-          pre += loc[q];
-        }
-        if (t == 0) BC[NB] = total;
-      }
-      gsync<NT>();
-      int* perm = K;  // the table keys are in registers now; reuse their LDS
-#pragma unroll
-      for (int j = 0; j < IPT; ++j)
-        if (ik[j] != kEmpty) perm[BC[ib[j]] + io[j]] = ik[j];
-      gsync<NT>();
-      int rk[IPT];
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        rk[j] = 0;
-        if (ik[j] != kEmpty) {
-          const int s0 = BC[ib[j]], s1 = BC[ib[j] + 1];
-          int r = s0;
-          for (int m = s0; m < s1; ++m) r += (perm[m] < ik[j]);
-          rk[j] = r;
-        }
-      }
-      gsync<NT>();
-#pragma unroll
-      for (int j = 0; j < IPT; ++j)
-        if (ik[j] != kEmpty) {
-          K[rk[j]] = ik[j];
-          H[rk[j]] = iv[j];
-        }
-      gsync<NT>();
-      const int n = BC[NB];
       const int64_t o0 = Cptr[row] - cbase;
       const int64_t nexp = Cptr[row + 1] - Cptr[row];
-      const int nw = (int)min((int64_t)n, nexp);
-      for (int m = t; m < nw; m += NT) {
-        Cidx[o0 + m] = K[m];
-        Cval[o0 + m] = H[m];
-      }
+      const int n = block_rank_emit<V, T, NT>(t, K, H, BC, sc[g], cmin, cmax, Cidx + o0, Cval + o0, nexp);
       bad = Group<NT>::reduce(bad, OpMax(), sc[g]);
       if (t == 0) {
         if (bad) atomicOr(err, kErrTableFull);
@@ -1304,105 +1321,236 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
                                  err);
 }
 
-// Long rows: dense LDS window of W columns (values + presence bitmap); emitted in column
-// order, so no sort is needed.
-template <typename V, int NT, int W, int G>
+// Cursor of one A entry of a long row in a column sweep: position in B and entries left of
+// its B row, and the column at that position (INT_MAX when the row is exhausted).
+struct LongState {
+  int64_t p;
+  int nc;
+  int rem;
+};
+
+// Lanes per A entry for a long row of nA entries in a group of NT threads: as many as divide
+// the group evenly over the entries (a power of two from 4 to 64).
+template <int NT>
+__device__ __forceinline__ int long_lanes(int64_t nA) {
+  int g = 64;
+  while (g > 4 && (int64_t)(NT / g) < nA) g >>= 1;
+  return g;
+}
+
+// Long rows, any column range: the row's columns are swept in windows of W columns, each
+// accumulated densely in LDS (values + presence bitmap) and emitted in column order, so no
+// sort is needed. A window starts at the smallest next column over the row's A entries, so
+// empty column ranges are skipped. Each A entry is walked by a subgroup of G lanes
+// (long_lanes). When every entry has its own subgroup (nA <= NT/G) the entry's cursor lives in
+// registers and the first B elements of the next window are loaded before the current window
+// is emitted; otherwise every entry has a LongState (in LDS up to kLongStates entries, else in
+// the workspace `state`, 16 bytes per A nonzero) and entries whose next column lies beyond the
+// window cost one LDS read per window.
+constexpr int kLongStates = 1024;
+
+template <typename V, int NT, int W>
 __global__ __launch_bounds__(NT) void k_numeric_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
-    const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx, const V* __restrict__ Aval,
-    const int64_t* __restrict__ Bptr, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+    const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
+    const V* __restrict__ Aval, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
     const int64_t* __restrict__ Cptr, int64_t cbase, int32_t* __restrict__ Cidx, V* __restrict__ Cval,
-    int64_t* __restrict__ cursor, int* __restrict__ err) {
+    LongState* __restrict__ state, int* __restrict__ err) {
   constexpr int NWORD = W / 32;
-  constexpr int WPT = NWORD / NT > 0 ? NWORD / NT : 1;
-  static_assert(NWORD % NT == 0 || NT % NWORD == 0, "shape");
+  constexpr int NW = NT / 64;
+  static_assert(NWORD <= NT && NT % 64 == 0, "one bitmap word per thread");
   __shared__ V dv[W];
   __shared__ uint32_t bm[NWORD];
-  __shared__ int sc[NT / 64];
-  const int t = threadIdx.x;
+  __shared__ LongState stl[kLongStates];
+  __shared__ int red_min[NW], red_cnt[NW];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int q = t; q < W; q += NT) dv[q] = V(0);
+  for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
+  __syncthreads();
+  int flags = 0;
   const int64_t beg = *pbeg, end = *pend;
   for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
     const int row = rows[slot];
-    const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
-    for (int s = t; s < W; s += NT) dv[s] = V(0);
-    for (int s = t; s < NWORD; s += NT) bm[s] = 0u;
-    int lo_l = INT_MAX;
-    for (int64_t ka = a0 + t; ka < a1; ka += NT) {
-      const int k = Aidx[ka];
-      const int64_t b0 = Bptr[k];
-      cursor[ka] = b0;
-      if (b0 < Bptr[k + 1]) lo_l = min(lo_l, Bidx[b0]);
-    }
-    int lo = Group<NT>::reduce(lo_l, OpMin(), sc);
+    const int64_t a0 = Aptr[row], a1 = Aptr[row + 1], nA = a1 - a0;
+    const int64_t c1 = Cptr[row + 1] - cbase;
     int64_t out = Cptr[row] - cbase;
-    const int64_t oend = Cptr[row + 1] - cbase;
-    int bad = 0;
+    const int G = long_lanes<NT>(nA);
+    const int NSG = NT / G, sg = t / G, lg = t & (G - 1), sgbase = lane & ~(G - 1);
+    const uint64_t submask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << sgbase;
+    const bool regs = nA <= NSG;
+    LongState* st = nA <= kLongStates ? stl : state + a0;
+    int64_t rp = 0;
+    int rrem = 0, rnc = INT_MAX, pc = INT_MAX;
+    V rva = V(0), pv = V(0);
+    int lmin = INT_MAX;
+    if (regs) {
+      if (sg < nA) {
+        rp = bstart[a0 + sg];
+        rrem = blen[a0 + sg];
+        rva = Aval[a0 + sg];
+        const int64_t at = rrem > 0 ? rp + min(lg, rrem - 1) : 0;
+        const int cv = Bidx[at];
+        const V bv = Bval[at];
+        pc = lg < rrem ? cv : INT_MAX;
+        pv = bv;
+      }
+      rnc = __shfl(pc, sgbase);  // subgroup-uniform: all lanes of a subgroup share sg
+      if (sg >= nA) rnc = INT_MAX;
+      lmin = rnc;
+    } else {
+      for (int64_t e = t; e < nA; e += NT) {
+        LongState x;
+        x.p = bstart[a0 + e];
+        x.rem = blen[a0 + e];
+        x.nc = x.rem > 0 ? Bidx[x.p] : INT_MAX;
+        st[e] = x;
+        lmin = min(lmin, x.nc);
+      }
+    }
+    {  // first window: smallest first column over the row
+      const int m = wave_reduce_dpp(lmin, OpMin());
+      if (lane == 0) red_min[wv] = m;
+      __syncthreads();
+      lmin = INT_MAX;
+#pragma unroll
+      for (int i = 0; i < NW; ++i) lmin = min(lmin, red_min[i]);
+      __syncthreads();
+    }
+    int lo = lmin;
     while (lo != INT_MAX) {
       const int64_t hi = (int64_t)lo + W;
-      for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
-        const int k = Aidx[ka];
-        const V va = Aval[ka];
-        int64_t p = cursor[ka];
-        const int64_t b1 = Bptr[k + 1];
-        while (p < b1) {
-          const int64_t q = p + (t % G);
-          const int c = (q < b1) ? Bidx[q] : INT_MAX;
-          const bool in = (int64_t)c < hi;
-          if (in) {
-            const int o = c - lo;
-            lds_add(&dv[o], va * Bval[q]);
-            atomicOr(&bm[o >> 5], 1u << (o & 31));
+      lmin = INT_MAX;
+      if (regs) {
+        if (rnc < hi) {  // subgroup-uniform
+          int c = pc;
+          V bv = pv;
+          for (;;) {
+            const bool in = c != INT_MAX && (int64_t)c < hi;  // INT_MAX: B row exhausted
+            if (in) {
+              const int o = c - lo;
+              lds_add(&dv[o], rva * bv);
+              atomicOr(&bm[o >> 5], 1u << (o & 31));
+            }
+            const int nin = __popcll(__ballot(in) & submask);
+            if (nin < G) {
+              rnc = __shfl(c, sgbase + nin);
+              rp += nin;
+              rrem -= nin;
+              break;
+            }
+            rp += G;
+            rrem -= G;
+            const int64_t at = rrem > 0 ? rp + min(lg, rrem - 1) : 0;
+            const int cv = Bidx[at];
+            bv = Bval[at];
+            c = lg < rrem ? cv : INT_MAX;
           }
-          const int nin = __popcll(__ballot(in) & subgroup_mask<G>());
-          p += nin;
-          if (nin < G) break;
+          // the next window's first elements, loaded while this window is emitted
+          const int64_t at = rrem > 0 ? rp + min(lg, rrem - 1) : 0;
+          const int cv = Bidx[at];
+          pv = Bval[at];
+          pc = lg < rrem ? cv : INT_MAX;
         }
-        if ((t % G) == 0) cursor[ka] = p;
+        lmin = rnc;
+      } else {
+        // 4 entries per subgroup at a time: their first loads are issued together
+        for (int64_t e0 = sg; e0 < nA; e0 += (int64_t)NSG * 4) {  // subgroup-uniform
+          LongState x[4];
+          int c[4];
+          V bv[4], va[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t e = e0 + (int64_t)u * NSG;
+            if (e < nA) x[u] = st[e];
+            else x[u] = LongState{0, INT_MAX, 0};
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            c[u] = INT_MAX;
+            bv[u] = V(0);
+            va[u] = V(0);
+            if (x[u].nc < hi) {  // subgroup-uniform
+              const int64_t at = x[u].rem > 0 ? x[u].p + min(lg, x[u].rem - 1) : 0;
+              const int cv = Bidx[at];
+              bv[u] = Bval[at];
+              va[u] = Aval[a0 + e0 + (int64_t)u * NSG];
+              c[u] = lg < x[u].rem ? cv : INT_MAX;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (x[u].nc < hi) {
+              int cu = c[u];
+              V bu = bv[u];
+              for (;;) {
+                const bool in = cu != INT_MAX && (int64_t)cu < hi;  // INT_MAX: B row exhausted
+                if (in) {
+                  const int o = cu - lo;
+                  lds_add(&dv[o], va[u] * bu);
+                  atomicOr(&bm[o >> 5], 1u << (o & 31));
+                }
+                const int nin = __popcll(__ballot(in) & submask);
+                if (nin < G) {
+                  x[u].nc = __shfl(cu, sgbase + nin);
+                  x[u].p += nin;
+                  x[u].rem -= nin;
+                  break;
+                }
+                x[u].p += G;
+                x[u].rem -= G;
+                const int64_t at = x[u].rem > 0 ? x[u].p + min(lg, x[u].rem - 1) : 0;
+                const int cv = Bidx[at];
+                bu = Bval[at];
+                cu = lg < x[u].rem ? cv : INT_MAX;
+              }
+              if (lg == 0) st[e0 + (int64_t)u * NSG] = x[u];
+            }
+            lmin = min(lmin, x[u].nc);
+          }
+        }
       }
+      __syncthreads();  // the window's products are in LDS
+      // emit in column order: one bitmap word per thread, positions by a group scan
+      uint32_t w = t < NWORD ? bm[t] : 0u;
+      const int cnt = __popc(w);
+      const int incl = wave_incl_scan_dpp(cnt);
+      const int wm = wave_reduce_dpp(lmin, OpMin());
+      if (lane == 63) red_cnt[wv] = incl;
+      if (lane == 0) red_min[wv] = wm;
       __syncthreads();
-      // emit the window in column order
-      uint32_t wv[WPT];
-      int cnt = 0;
+      int base = 0, tot = 0, nlo = INT_MAX;
 #pragma unroll
-      for (int q = 0; q < WPT; ++q) {
-        const int w = t * WPT + q;
-        wv[q] = (w < NWORD) ? bm[w] : 0u;
-        cnt += __popc(wv[q]);
+      for (int i = 0; i < NW; ++i) {
+        const int ci = red_cnt[i];
+        base += i < wv ? ci : 0;
+        tot += ci;
+        nlo = min(nlo, red_min[i]);
       }
-      int tot;
-      int pos = Group<NT>::exscan(cnt, tot, sc);
-#pragma unroll
-      for (int q = 0; q < WPT; ++q) {
-        const int w = t * WPT + q;
-        uint32_t m = wv[q];
-        while (m) {
-          const int b = __ffs(m) - 1;
-          m &= m - 1u;
-          const int o = w * 32 + b;
-          if (out + pos < oend) {
-            Cidx[out + pos] = lo + o;
-            Cval[out + pos] = dv[o];
+      if (w) {
+        int64_t pos = out + base + incl - cnt;
+        bm[t] = 0u;
+        while (w) {
+          const int bb = __ffs(w) - 1;
+          w &= w - 1u;
+          const int o = t * 32 + bb;
+          if (pos < c1) {
+            Cidx[pos] = lo + o;
+            Cval[pos] = dv[o];
           } else {
-            bad = 1;
+            flags |= kErrWindow;
           }
           dv[o] = V(0);
           ++pos;
         }
-        if (w < NWORD) bm[w] = 0u;
       }
       out += tot;
-      int nl = INT_MAX;
-      for (int64_t ka = a0 + t; ka < a1; ka += NT) {
-        const int k = Aidx[ka];
-        const int64_t p = cursor[ka];
-        if (p < Bptr[k + 1]) nl = min(nl, Bidx[p]);
-      }
-      lo = Group<NT>::reduce(nl, OpMin(), sc);  // its barriers also fence the window clear
+      lo = nlo;
+      __syncthreads();  // cleared before the next window's products; reduction slots free
     }
-    bad = Group<NT>::reduce(bad, OpMax(), sc);
-    if (t == 0 && (bad || out != oend)) atomicOr(err, kErrWindow);
-    __syncthreads();
+    if (out != c1) flags |= kErrWindow;
   }
+  flags = wave_reduce_dpp(flags, OpOr());
+  if (lane == 0 && flags) atomicOr(err, flags);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1652,8 +1800,8 @@ static int grid_for(int64_t nrows, int gpb, int grid, const void* f = nullptr, i
 
 constexpr int kLongNT = 256;
 constexpr int kSymLongW = 131072;  // bits per symbolic window (16 KB bitmap)
-constexpr int kNumLongW64 = 8192;  // FP64 dense window (64 KB + 1 KB bitmap)
-constexpr int kNumLongW32 = 16384; // FP32 dense window (64 KB + 2 KB bitmap)
+constexpr int kNumLongW64 = 4096;  // FP64 dense window (32 KB + 512 B bitmap; 3 workgroups per CU)
+constexpr int kNumLongW32 = 8192;  // FP32 dense window (32 KB + 1 KB bitmap)
 constexpr int kBwdLongW64 = 8192;
 constexpr int kBwdLongW32 = 16384;
 constexpr int kLongG = 16;
@@ -1707,8 +1855,8 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
   if (k.T == 0) {
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int W = sizeof(V) == 8 ? kNumLongW64 : kNumLongW32;
-    hipLaunchKernelGGL((k_numeric_long<V, kLongNT, W, kLongG>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
-                       A.idx, Av, B.ptr, B.idx, Bv, Cptr, cbase, Cidx, Cval, cursor, err);
+    hipLaunchKernelGGL((k_numeric_long<V, kLongNT, W>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
+                       A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, (LongState*)cursor, err);
     return;
   }
 #define X(T_, NT_, G_)                                                                                           \

@@ -108,6 +108,30 @@ def test_all_categories(gpu_ok, dtype):
     assert sum(1 for x in info["num_rows"][1:] if x > 0) >= 6, info["num_rows"]
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_long_rows_wide_column_range(gpu_ok, dtype):
+    """Long rows (nnz_i > 4096) of 30-36 A entries over a 5,000,000-column range: the numeric
+    window sweep (dense column windows, cursors in registers) and the symbolic bitmap windows."""
+    rng = np.random.default_rng(21)
+    ncols = 5_000_000
+    rows_b, cols_b = [], []
+    for i in range(64):
+        c = rng.choice(ncols, size=300, replace=False)
+        rows_b.append(np.full(c.size, i))
+        cols_b.append(c)
+    B = sp.csr_matrix((rng.standard_normal(64 * 300), (np.concatenate(rows_b), np.concatenate(cols_b))),
+                      shape=(64, ncols))
+    rows_a, cols_a = [], []
+    for i in range(50):
+        c = rng.choice(64, size=30 + (i % 7), replace=False)
+        rows_a.append(np.full(c.size, i))
+        cols_a.append(c)
+    A = sp.csr_matrix((rng.standard_normal(sum(x.size for x in cols_a)),
+                       (np.concatenate(rows_a), np.concatenate(cols_a))), shape=(50, 64))
+    C, t, info = check_product(csr_sp(A, dtype), csr_sp(B, dtype), what="wide long rows", block_rows=512)
+    assert info["num_rows"][-1] == 50 and info["sym_rows"][-1] == 50, (info["num_rows"], info["sym_rows"])
+
+
 def test_block_rows_do_not_change_result(gpu_ok):
     A = csr_sp(_skewed_matrix(3000, 3000, {5: 900}, 21))
     ref = reference(A)
