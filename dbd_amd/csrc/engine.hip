@@ -169,13 +169,17 @@ struct WriteJob {
   int64_t off;  // element offset of the chunk in C
 };
 
+// File writers of DBD_MODE_IO: one thread per output file (Idx.output, Val.output), each
+// writing the chunks in order at their final offsets, so every file is written front to back
+// by a single writer (the filesystem serialises writes to one file anyway). A pinned ring
+// slot is free again when both files have written it.
 struct Writer {
   int fd_idx = -1, fd_val = -1;
   size_t vsz = 8;
   std::mutex mu;
   std::condition_variable cv_job, cv_free;
-  std::deque<WriteJob> q;
-  std::vector<int> slot_busy;
+  std::deque<WriteJob> q[2];
+  std::vector<int> slot_busy;  // file writes still pending per slot (0, 1 or 2)
   std::vector<std::thread> th;
   bool stop = false;
   std::atomic<bool> failed{false};
@@ -183,36 +187,36 @@ struct Writer {
   double busy_s = 0;
   double bytes = 0;
 
-  void start(int nthreads, int nslots) {
+  void start(int nslots) {
     slot_busy.assign(nslots, 0);
     stop = false;
     failed = false;
-    for (int i = 0; i < nthreads; ++i) th.emplace_back([this] { loop(); });
+    for (int f = 0; f < 2; ++f) th.emplace_back([this, f] { loop(f); });
   }
-  void loop() {
+  void loop(int f) {
     for (;;) {
       WriteJob j;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_job.wait(lk, [&] { return stop || !q.empty(); });
-        if (q.empty()) return;
-        j = q.front();
-        q.pop_front();
+        cv_job.wait(lk, [&] { return stop || !q[f].empty(); });
+        if (q[f].empty()) return;
+        j = q[f].front();
+        q[f].pop_front();
       }
       double t0 = now_s();
       bool ok = hipEventSynchronize(j.ready) == hipSuccess;
-      ok = ok && pw(fd_idx, j.idx, 4 * (size_t)j.n, (off_t)(4 * j.off));
-      ok = ok && pw(fd_val, j.val, vsz * (size_t)j.n, (off_t)(vsz * (size_t)j.off));
+      const size_t es = f == 0 ? 4 : vsz;
+      ok = ok && pw(f == 0 ? fd_idx : fd_val, f == 0 ? j.idx : j.val, es * (size_t)j.n, (off_t)(es * (size_t)j.off));
       double t1 = now_s();
       {
         std::lock_guard<std::mutex> lk(mu);
         busy_s += t1 - t0;
-        bytes += (double)(4 + vsz) * (double)j.n;
+        bytes += (double)es * (double)j.n;
         if (!ok && !failed) {
           failed = true;
           why = strerror(errno);
         }
-        slot_busy[j.slot] = 0;
+        --slot_busy[j.slot];
       }
       cv_free.notify_all();
     }
@@ -234,14 +238,15 @@ struct Writer {
   void acquire(int slot) {
     std::unique_lock<std::mutex> lk(mu);
     cv_free.wait(lk, [&] { return slot_busy[slot] == 0 || failed; });
-    slot_busy[slot] = 1;
+    slot_busy[slot] = 2;
   }
   void push(const WriteJob& j) {
     {
       std::lock_guard<std::mutex> lk(mu);
-      q.push_back(j);
+      q[0].push_back(j);
+      q[1].push_back(j);
     }
-    cv_job.notify_one();
+    cv_job.notify_all();
   }
   void finish() {
     {
@@ -618,7 +623,7 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
       wr.vsz = c->vsz;
       if (ftruncate(wr.fd_idx, (off_t)(4 * nnzC)) != 0 || ftruncate(wr.fd_val, (off_t)(c->vsz * nnzC)) != 0)
         return set_error(DBD_ERR_IO, "cannot size output files: %s", strerror(errno));
-      wr.start(c->o.writer_threads > 0 ? c->o.writer_threads : 4, nslots);
+      wr.start(nslots);
     }
   }
   // chunk loop

@@ -76,7 +76,8 @@ typedef struct dbd_opts {
   int32_t verbose;           /* 1: print the reference's progress lines to stdout */
   const char* out_dir;       /* DBD_MODE_IO: directory of Ptr/Idx/Val.output (NULL -> cwd) */
   int32_t ptr64_file;        /* 1: always write Ptr.output as int64 (else int32 when nnz(C) < 2^31) */
-  int32_t writer_threads;    /* file writer threads; 0 -> 4 */
+  int32_t writer_threads;    /* pinned ring depth in DBD_MODE_IO is writer_threads + 1 chunk
+                                slots (0 -> 4); each output file has one writer thread */
   int32_t chunks_hint;       /* minimum number of output chunks in D2H/IO modes; 0 -> 8 */
   int32_t retain_result;     /* 1: keep whole C (device in COMPUTE mode, host in D2H mode) if it fits */
 } dbd_opts;
