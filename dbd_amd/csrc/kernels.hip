@@ -921,6 +921,26 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
   }
 }
 
+// Cursor of one A entry of a long row in a column sweep: position in B and entries left of
+// its B row, and the column at that position (INT_MAX when the row is exhausted).
+struct LongState {
+  int64_t p;
+  int nc;
+  int rem;
+};
+
+// Lanes per A entry for a long row of nA entries in a group of NT threads: as many as divide
+// the group evenly over the entries (a power of two from 4 to 64).
+template <int NT>
+__device__ __forceinline__ int long_lanes(int64_t nA) {
+  int g = 64;
+  while (g > 4 && (int64_t)(NT / g) < nA) g >>= 1;
+  return g;
+}
+
+// Entries whose cursor state a long-row kernel keeps in LDS (more go to the workspace).
+constexpr int kLongStates = 1024;
+
 // Rows too long for an LDS table: sweep the row's columns in windows of W bits; one cursor
 // per A entry (cursor[] is indexed like A's nonzeros) walks its sorted B row.
 template <int NT, int W, int G>
@@ -1101,8 +1121,8 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
 // then the rank inside a bucket by comparison. The table's key and value LDS are reused.
 // Leaves the table (K, H) unspecified and BC dirty; the caller clears them.
 template <typename V, int T, int NT>
-__device__ int block_rank_emit(int t, int* K, V* H, int* BC, int* sc, int cmin, int cmax, int32_t* __restrict__ Cidx,
-                               V* __restrict__ Cval, int64_t limit) {
+__device__ __forceinline__ int block_rank_emit(int t, int* K, V* H, int* BC, int* sc, int cmin, int cmax,
+                                               int32_t* __restrict__ Cidx, V* __restrict__ Cval, int64_t limit) {
   constexpr int IPT = T / NT;
   constexpr int NB = T / 2;
   constexpr int PER = (NB + NT - 1) / NT;
@@ -1321,23 +1341,6 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
                                  err);
 }
 
-// Cursor of one A entry of a long row in a column sweep: position in B and entries left of
-// its B row, and the column at that position (INT_MAX when the row is exhausted).
-struct LongState {
-  int64_t p;
-  int nc;
-  int rem;
-};
-
-// Lanes per A entry for a long row of nA entries in a group of NT threads: as many as divide
-// the group evenly over the entries (a power of two from 4 to 64).
-template <int NT>
-__device__ __forceinline__ int long_lanes(int64_t nA) {
-  int g = 64;
-  while (g > 4 && (int64_t)(NT / g) < nA) g >>= 1;
-  return g;
-}
-
 // Long rows, any column range: the row's columns are swept in windows of W columns, each
 // accumulated densely in LDS (values + presence bitmap) and emitted in column order, so no
 // sort is needed. A window starts at the smallest next column over the row's A entries, so
@@ -1347,8 +1350,6 @@ __device__ __forceinline__ int long_lanes(int64_t nA) {
 // is emitted; otherwise every entry has a LongState (in LDS up to kLongStates entries, else in
 // the workspace `state`, 16 bytes per A nonzero) and entries whose next column lies beyond the
 // window cost one LDS read per window.
-constexpr int kLongStates = 1024;
-
 template <typename V, int NT, int W>
 __global__ __launch_bounds__(NT) void k_numeric_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
