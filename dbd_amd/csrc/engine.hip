@@ -483,13 +483,16 @@ dbd_status numeric_blocks(dbd_ctx* c, int64_t b0, int64_t b1, const int64_t* Cpt
   const int64_t w0 = r0 / 256, w1 = (r1 + 255) / 256;  // bin tiles (BR is a multiple of 256)
   const CatTable& nt = c->tb.num_ct;
   const int64_t* offs = c->offs_num.as<int64_t>();
+  // few duplicate products overall (nnzC >= 90% of the products): rows of distinct products
+  // take the numeric kernels' register path
+  const bool distinct = c->info.nnz >= 0 && c->info.nnz >= c->info.products - c->info.products / 10;
   for (int k = 1; k < nt.n; ++k) {
     const int64_t i0 = (int64_t)k * c->nwg + w0, i1 = (int64_t)k * c->nwg + w1;
     const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
     if (n <= 0) continue;
     if (c->tb.num_k[k].T == 0) CHK(ensure_cursor(c));
     launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
-                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st);
+                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st, distinct);
   }
   return DBD_OK;
 }

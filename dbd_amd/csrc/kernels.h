@@ -66,10 +66,13 @@ int symbolic_groups_per_wg(const KernelShape& k);
 void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                      int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
                      int* err, hipStream_t s);
+// distinct: most products of the problem are distinct columns of their row (nnzC close to the
+// product count); the one-wave shapes then take rows whose products are all distinct through a
+// register path instead of the hash table.
 void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                     const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                     const int64_t* Cptr, int64_t cbase, int32_t* Cidx, void* Cval, int64_t* cursor, int* err,
-                    hipStream_t s);
+                    hipStream_t s, bool distinct = false);
 void launch_backward(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                      const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                      const int64_t* Cptr, const int32_t* Cidx, const void* dC, void* dA, void* dB, int64_t* cursor,

@@ -1019,19 +1019,18 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 // numeric: accumulate in an LDS hash, sort by a monotone bucket map + in-bucket rank,
 // emit coalesced
 // ------------------------------------------------------------------------------------
-// Rank the live entries of a one-wave row table (T slots, at most T/2 live) in column order
-// without moving them. Each lane takes the keys of its IPT slots into registers; the key table
-// is then reused for the bucket counters (lower half) and the bucket permutation (upper half).
-// The bucket map b(x) = floor(x * mul / 2^32), x = c - cmin (or b = x when the column range
-// fits the buckets) is monotone, so a key's rank is the start of its bucket plus the number of
-// smaller keys in the bucket; mul only has to keep b below NB, so it is formed in single
-// precision and rounded down. LDS reads of all slots are issued before their results are used.
-// rk[q] = rank of slot slot_of(t, q), -1 if empty. Returns the number of live entries (more than
-// T/2 means the table was overfilled; ranks are then not exact and the caller reports it).
-template <int T>
-__device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax, int (&kk)[T / 64],
-                                               int (&rk)[T / 64]) {
-  constexpr int IPT = T / 64;
+// Rank distinct keys held in registers (IPT per lane, kEmpty = none, at most T/2 in all) in
+// column order, using the T ints at K (free for the call) for the bucket counters (lower half)
+// and the bucket permutation (upper half). The bucket map b(x) = floor(x * mul / 2^32),
+// x = c - cmin (or b = x when the column range fits the buckets) is monotone, so a key's rank is
+// the start of its bucket plus the number of smaller keys in the bucket; mul only has to keep b
+// below NB, so it is formed in single precision and rounded down. LDS reads of all keys are
+// issued before their results are used. rk[q] = rank of kk[q], -1 if empty. Returns the number
+// of keys (more than T/2 means a table was overfilled; ranks are then not exact and the caller
+// reports it).
+template <int T, int IPT>
+__device__ __forceinline__ int wave_rank_keys(int t, int* K, int cmin, int cmax, const int (&kk)[IPT],
+                                              int (&rk)[IPT]) {
   constexpr int H2 = T / 2;
   constexpr int NB = H2 - 1;           // buckets; BC[NB] receives the total
   constexpr int PER = (H2 + 63) / 64;  // counters per lane in the scan
@@ -1042,9 +1041,6 @@ __device__ __forceinline__ int wave_rank_table(int t, int* K, int cmin, int cmax
       direct ? 0u : (uint32_t)((float)NB * 4294967296.0f / (float)range * 0.99999f);
   int* BC = K;
   int* perm = K + H2;
-#pragma unroll
-  for (int q = 0; q < IPT; ++q) kk[q] = K[slot_of<T>(t, q)];
-  wave_sync();
 #pragma unroll
   for (int q = 0; q < PER; ++q)
     if (t * PER + q < H2) BC[t * PER + q] = 0;
@@ -1201,7 +1197,7 @@ __device__ __forceinline__ int block_rank_emit(int t, int* K, V* H, int* BC, int
   return n;
 }
 
-template <typename V, int T, int NT, int G>
+template <typename V, int T, int NT, int G, bool FAST = false>
 __device__ __forceinline__ void numeric_hash_body(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
     const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
@@ -1226,7 +1222,7 @@ __device__ __forceinline__ void numeric_hash_body(
   const int64_t beg = *pbeg, end = *pend;
   if constexpr (WAVE) {
     // one wave per row, rows software-pipelined (RowPipe); the table is ranked by
-    // wave_rank_table while the next row's descriptors are in flight
+    // wave_rank_keys while the next row's descriptors are in flight
     const int64_t s0 = beg + xcd_block() * GPB + g;
     if (s0 >= end) return;
     RowPipe<G, true, V> rp;
@@ -1234,8 +1230,70 @@ __device__ __forceinline__ void numeric_hash_body(
     Step<4, V> st0;  // first gather step of the current row, issued during the previous row
     gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.a1 - rp.a0), rp.d, Bidx, Bval);
     int flags = 0;   // error bits of all rows of this wave, reported once after the loop
+    constexpr int STEP = G == 16 ? 16 : 4;  // A entries covered by the first gather step
     for (;;) {
       rp.prefetch(t, rows, Aptr, Cptr, bstart, blen, Aval);
+      // Rows whose products are all distinct (P_i equals the symbolic count nnz_i) and are all
+      // in the first gather step (at most STEP A entries, B rows of at most G entries) skip the
+      // hash table: their products are ranked straight from registers.
+      bool fast = false;
+      if (FAST && rp.a1 - rp.a0 <= STEP) {
+        const bool inrow = desc_slot<G>(t) < rp.a1 - rp.a0;
+        const bool longb = __builtin_amdgcn_ballot_w64(inrow && rp.d.len > G) != 0;
+        const int P = wave_reduce_dpp(inrow ? min(rp.d.len, G + 1) : 0, OpAdd());
+        fast = !longb && (int64_t)P == rp.c1 - rp.c0 && P <= T / 2;
+      }
+      if (fast) {
+        const int64_t nA = rp.a1 - rp.a0;
+        const int64_t nexp = rp.c1 - rp.c0;
+        const int sg = t / G, lg = t % G;
+        const int cnt = (int)nA;
+        int kc[4], rk[4];
+        V kv[4];
+        int lmin = INT_MAX, lmax = -1;
+        static_for<4>([&](auto uc) {
+          constexpr int u = decltype(uc)::value;
+          const int e = step_entry<G, 0>(0, u, sg);
+          const int lv = ebcast<G, u>(rp.d.len, u);
+          const V pv = ebcastv<G, u>(rp.d.va, u);
+          const bool has = e < cnt && lg < lv;
+          kc[u] = has ? st0.c[u] : kEmpty;
+          kv[u] = pv * st0.bv[u];
+          lmin = has ? min(lmin, kc[u]) : lmin;
+          lmax = has ? max(lmax, kc[u]) : lmax;
+        });
+        // the next row's first gathers, in flight while this row is ranked and stored
+        gather_step<G, 4, true, V>(st0, t, 0, (int)min((int64_t)64, rp.na1 - rp.na0), rp.nd, Bidx, Bval);
+        const int cmin = wave_reduce_dpp(lmin, OpMin());
+        const int cmax = wave_reduce_dpp(lmax, OpMax());
+        const int n = wave_rank_keys<T, 4>(t, K, cmin, cmax, kc, rk);
+        flags |= ((int64_t)n != nexp) ? kErrCountMismatch : 0;
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (rk[u] >= 0) {
+            K[rk[u]] = kc[u];
+            H[rk[u]] = kv[u];
+          }
+        wave_sync();
+        // fixed-count unconditional coalesced stores, as below
+        const int64_t o0 = rp.c0 - cbase;
+        const int nw = (int)min((int64_t)n, nexp);
+        int32_t* di = nw > 0 ? Cidx + o0 : (int32_t*)(err + 4);
+        V* dv = nw > 0 ? Cval + o0 : (V*)(err + 8);
+        const int last = max(nw, 1) - 1;
+#pragma unroll
+        for (int j = 0; j < (T / 2 + 63) / 64; ++j) {
+          const int m = min(t + 64 * j, last);
+          const int key = K[m];
+          const V val = H[m];
+          di[m] = key;
+          dv[m] = val;
+        }
+        wave_sync();
+        if (!rp.advance()) break;
+        continue;
+      }
       clear_table<T, V>(t, K, H);
       wave_sync();
       HashSink<T, V, true> sk{K, H};
@@ -1247,7 +1305,10 @@ __device__ __forceinline__ void numeric_hash_body(
       const int cmin = wave_reduce_dpp(sk.cmin, OpMin());
       const int cmax = wave_reduce_dpp(sk.cmax, OpMax());
       int kk[IPT], rk[IPT];
-      const int n = wave_rank_table<T>(t, K, cmin, cmax, kk, rk);
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) kk[q] = K[slot_of<T>(t, q)];
+      wave_sync();
+      const int n = wave_rank_keys<T, IPT>(t, K, cmin, cmax, kk, rk);
       const int64_t o0 = rp.c0 - cbase;
       const int64_t nexp = rp.c1 - rp.c0;
       flags |= (sk.bad || n > T / 2) ? kErrTableFull : 0;
@@ -1316,9 +1377,13 @@ __device__ __forceinline__ void numeric_hash_body(
 }
 
 // Occupancy target of the one-wave-per-row numeric kernel (waves per SIMD); the compiler fits
-// its registers to it. The workgroup-per-row shapes are limited by LDS instead.
+// its registers to it. The workgroup-per-row shapes are limited by LDS instead. The variant with
+// the register path for rows of distinct products needs more registers: 4 waves per SIMD.
 #ifndef DBD_NUMERIC_WAVES_PER_EU
 #define DBD_NUMERIC_WAVES_PER_EU 5
+#endif
+#ifndef DBD_NUMERIC_FAST_WAVES_PER_EU
+#define DBD_NUMERIC_FAST_WAVES_PER_EU 4
 #endif
 
 #define DBD_NUMERIC_ARGS                                                                                     \
@@ -1333,6 +1398,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DBD_NUMERIC
 k_numeric_wave(DBD_NUMERIC_ARGS) {
   numeric_hash_body<V, T, NT, G>(rows, pbeg, pend, Aptr, bstart, blen, Aval, Bidx, Bval, Cptr, cbase, Cidx, Cval,
                                  err);
+}
+
+// Same rows, with the register path for rows whose products are all distinct (chosen when the
+// problem has few duplicate products).
+template <typename V, int T, int NT, int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DBD_NUMERIC_FAST_WAVES_PER_EU))) void
+k_numeric_wave_fast(DBD_NUMERIC_ARGS) {
+  numeric_hash_body<V, T, NT, G, true>(rows, pbeg, pend, Aptr, bstart, blen, Aval, Bidx, Bval, Cptr, cbase, Cidx,
+                                       Cval, err);
 }
 
 template <typename V, int T, int NT, int G>
@@ -1849,7 +1923,8 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
 template <typename V>
 static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                              int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr,
-                             int64_t cbase, int32_t* Cidx, V* Cval, int64_t* cursor, int* err, hipStream_t s) {
+                             int64_t cbase, int32_t* Cidx, V* Cval, int64_t* cursor, int* err, hipStream_t s,
+                             bool distinct) {
   const int gpb = symbolic_groups_per_wg(k);
   const V* Av = (const V*)A.val;
   const V* Bv = (const V*)B.val;
@@ -1863,9 +1938,15 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
 #define X(T_, NT_, G_)                                                                                           \
   if (k.T == T_ && k.NT == NT_ && k.G == G_) {                                                                 \
     if constexpr (NT_ == 64) {                                                                                 \
-      const dim3 gr(grid_for(nrows_hint, gpb, grid, (const void*)k_numeric_wave<V, T_, NT_, G_>, 256));        \
-      hipLaunchKernelGGL((k_numeric_wave<V, T_, NT_, G_>), gr, dim3(256), 0, s, rows, pbeg, pend, A.ptr,       \
-                         A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                      \
+      if (distinct) {                                                                                          \
+        const dim3 gr(grid_for(nrows_hint, gpb, grid, (const void*)k_numeric_wave_fast<V, T_, NT_, G_>, 256)); \
+        hipLaunchKernelGGL((k_numeric_wave_fast<V, T_, NT_, G_>), gr, dim3(256), 0, s, rows, pbeg, pend,       \
+                           A.ptr, A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);             \
+      } else {                                                                                                 \
+        const dim3 gr(grid_for(nrows_hint, gpb, grid, (const void*)k_numeric_wave<V, T_, NT_, G_>, 256));      \
+        hipLaunchKernelGGL((k_numeric_wave<V, T_, NT_, G_>), gr, dim3(256), 0, s, rows, pbeg, pend, A.ptr,     \
+                           A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, err);                    \
+      }                                                                                                        \
     } else {                                                                                                    \
       const dim3 gr(grid_for(nrows_hint, gpb, grid));                                                          \
       hipLaunchKernelGGL((k_numeric_block<V, T_, NT_, G_>), gr, dim3(NT_), 0, s, rows, pbeg, pend, A.ptr,      \
@@ -1880,13 +1961,13 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
 
 void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                     int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr, int64_t cbase,
-                    int32_t* Cidx, void* Cval, int64_t* cursor, int* err, hipStream_t s) {
+                    int32_t* Cidx, void* Cval, int64_t* cursor, int* err, hipStream_t s, bool distinct) {
   if (dtype == 1)
     launch_numeric_t<double>(k, rows, pbeg, pend, nrows_hint, grid, A, B, Cptr, cbase, Cidx, (double*)Cval, cursor,
-                             err, s);
+                             err, s, distinct);
   else
     launch_numeric_t<float>(k, rows, pbeg, pend, nrows_hint, grid, A, B, Cptr, cbase, Cidx, (float*)Cval, cursor,
-                            err, s);
+                            err, s, distinct);
 }
 
 template <typename V>

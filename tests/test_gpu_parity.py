@@ -210,6 +210,58 @@ def test_er_16_per_row_sampled_full_size(gpu_ok):
     np.testing.assert_allclose(Csp @ x, Asp @ (Bsp @ x), rtol=1e-9, atol=1e-9)
 
 
+def _mostly_distinct(m, k, n, seed, dtype):
+    """A (m x k) * B (k x n) where most rows' products fall on distinct columns (B rows of 8
+    random columns out of n = 10^6), plus rows that exercise the hash path inside the same
+    kernel: rows with two B rows from a block of identical patterns (duplicate products), rows
+    selecting B rows of 20 entries (longer than a gather step), rows of 17-24 entries."""
+    rng = np.random.default_rng(seed)
+    bl = np.full(k, 8)
+    bl[100:200] = 20
+    bptr = np.zeros(k + 1, np.int64)
+    bptr[1:] = np.cumsum(bl)
+    bidx = np.empty(bptr[-1], np.int32)
+    for r in range(k):
+        cols = np.arange(8) if r < 100 else rng.choice(n, bl[r], replace=False)
+        bidx[bptr[r]:bptr[r + 1]] = np.sort(cols)
+    B = dbd_amd.CSR(bptr, bidx, rng.standard_normal(bptr[-1]).astype(dtype), (k, n))
+    rows = []
+    for i in range(m):
+        kind = rng.random()
+        if kind < 0.03:
+            e = np.concatenate([rng.choice(100, 2, replace=False), rng.choice(np.arange(200, k), 10, replace=False)])
+        elif kind < 0.06:
+            e = np.concatenate([rng.choice(np.arange(100, 200), 3, replace=False),
+                                rng.choice(np.arange(200, k), 8, replace=False)])
+        elif kind < 0.08:
+            e = rng.choice(np.arange(200, k), int(rng.integers(17, 25)), replace=False)
+        else:
+            e = rng.choice(np.arange(200, k), int(rng.integers(1, 17)), replace=False)
+        rows.append(np.sort(e))
+    aptr = np.zeros(m + 1, np.int64)
+    aptr[1:] = np.cumsum([len(e) for e in rows])
+    aidx = np.concatenate(rows).astype(np.int32)
+    A = dbd_amd.CSR(aptr, aidx, rng.standard_normal(aptr[-1]).astype(dtype), (m, k))
+    return A, B
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_distinct_products_register_path(gpu_ok, dtype):
+    """Problems with few duplicate products run the one-wave numeric shapes with the register
+    path for rows of all-distinct products; rows with duplicates, long B rows or many entries
+    take the hash path of the same kernel. Every candidate table (G = 16 and G = 64 shapes)."""
+    A, B = _mostly_distinct(20000, 5000, 1_000_000, 77, dtype)
+    ref = reference(A, B)
+    for tid in range(dbd_amd.Context.table_count()):
+        with dbd_amd.Context(block_rows=512, dtype=dtype) as ctx:
+            ctx.set_table(tid)
+            ctx.set_operands(A, B)
+            ctx.run("compute")
+            info = ctx.plan_info()
+            assert info["nnz"] >= 0.9 * info["products"] > 0
+            compare(ctx.result(), ref, dtype, f"table {tid}")
+
+
 def test_autotuned_tables_give_identical_results(gpu_ok):
     """Property: every candidate category table (the autotuner's search space) gives the same
     structure and oracle-exact values; the tuned choice is one of them."""
