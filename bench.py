@@ -259,7 +259,8 @@ def main():
     achieved = nb_bytes / (num_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": load_traffic(),
-            "kernel": "k_numeric_hash<double,512,64,16> (numeric pass, %d category launch(es))" % ncat,
+            "kernel": "%s<double,512,64,16> (numeric pass, %d category launch(es))"
+                      % ("k_numeric_wave_fast" if 10 * nnz2 >= 9 * P else "k_numeric_wave", ncat),
             "kernel_ms": round(num_ms, 4), "algorithmic_bytes": nb_bytes,
             "symbolic_pass_ms": round(sym_ms, 4),
             "symbolic_achieved_GBps": round(symbolic_bytes(N, nnzA, P) / (sym_ms * 1e-3) / 1e9, 1)}
