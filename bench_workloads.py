@@ -3,8 +3,9 @@
     web     web-like 281,903^2 C = A*A, FP64, 4096-row blocks (literal Zipf 2.1 out-degree and
             the nnz-matched exponent 1.66), all three windows, full-size oracle on the host
     rmat22  R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05), C = A*A, FP64, 512-row blocks, 64 GB
-            device budget: kernel window over the whole product (streamed in chunks), D2H and
-            file windows on a bounded leading row range
+            device budget: kernel window over the whole product (streamed in chunks), the whole
+            product copied to pinned host memory (mode token d2hfull), D2H and file windows on
+            the trailing 2^19 rows
     batch   512 independent C_i = A_i * B_i (FP32; rows log-uniform in [1e3, 2e5], mean
             nnz/row U[4, 64] with Zipf(1.8) row skew): one captured graph vs per-matrix plain runs
     fwdbwd  R-MAT scale 18 A (row-normalised adjacency + self loops) times an independent R-MAT
@@ -146,10 +147,22 @@ def run_rmat22(args):
                       "num_rows_per_category": info["num_rows"]}}
     log(json.dumps(res["kernel"]))
     ctx.close()
+    modes = [m for m in args.modes.split(",") if m]
+    if "d2hfull" in modes:
+        # the whole product with C copied to pinned host memory (chunks through a 64 GB ring):
+        # the end-to-end window whose sink is host memory, the CPU path's own output form
+        c2 = dbd_amd.Context(block_rows=512, dtype=np.float64, device_gib=64, host_gib=64, retain=False)
+        c2.set_operands(A)
+        c2.plan()
+        t2, tm2 = _timed(c2, "d2h", 1, 0)
+        res["d2h_full"] = {"ms": round(t2 * 1e3, 2), "gflops": round(2 * P / t2 / 1e9, 2), "chunks": tm2["chunks"],
+                           "d2h_bytes": tm2["d2h_bytes"],
+                           "d2h_GBps": round(tm2["d2h_bytes"] / t2 / 1e9, 2)}
+        log(json.dumps(res["d2h_full"]))
+        c2.close()
     # D2H and file windows on a bounded contiguous row range (C of the full product is ~1 TB):
     # the last 2^19 rows, ~2.5e9 products (R-MAT's leading rows are its densest: the first
     # 2^18 rows alone hold 29% of all products)
-    modes = [m for m in args.modes.split(",") if m]
     if "d2h" in modes or "io" in modes:
         r = min(A.shape[0], 1 << 19)
         r0 = A.shape[0] - r
@@ -172,6 +185,12 @@ def run_rmat22(args):
         dt = time.perf_counter() - t0
         res["cpu_sample"] = {"rows": int(sel.size), "products": int(Prow[sel].sum()), "s": round(dt, 3),
                              "gflops": round(2 * Prow[sel].sum() / dt / 1e9, 3), "threads": _threads()}
+        # whole-product CPU time at the sample's rate (rows drawn uniformly at random)
+        cpu_s = 2 * P / (res["cpu_sample"]["gflops"] * 1e9)
+        res["cpu_sample"]["whole_product_s_est"] = round(cpu_s, 1)
+        if "d2h_full" in res:
+            res["end_to_end_d2h_vs_cpu"] = round(cpu_s / (res["d2h_full"]["ms"] * 1e-3), 2)
+        res["kernel_vs_cpu"] = round(cpu_s / (res["kernel"]["ms"] * 1e-3), 2)
     print(json.dumps(res))
 
 
