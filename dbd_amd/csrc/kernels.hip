@@ -938,8 +938,6 @@ __device__ __forceinline__ int long_lanes(int64_t nA) {
   return g;
 }
 
-// Entries whose cursor state a long-row kernel keeps in LDS (more go to the workspace).
-constexpr int kLongStates = 1024;
 
 // Rows too long for an LDS table: sweep the row's columns in windows of W bits; one cursor
 // per A entry (cursor[] is indexed like A's nonzeros) walks its sorted B row.
@@ -1415,33 +1413,67 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
                                  err);
 }
 
-// Long rows, any column range: the row's columns are swept in windows of W columns, each
-// accumulated densely in LDS (values + presence bitmap) and emitted in column order, so no
-// sort is needed. A window starts at the smallest next column over the row's A entries, so
-// empty column ranges are skipped. Each A entry is walked by a subgroup of G lanes
-// (long_lanes). When every entry has its own subgroup (nA <= NT/G) the entry's cursor lives in
-// registers and the first B elements of the next window are loaded before the current window
-// is emitted; otherwise every entry has a LongState (in LDS up to kLongStates entries, else in
-// the workspace `state`, 16 bytes per A nonzero) and entries whose next column lies beyond the
-// window cost one LDS read per window.
-template <typename V, int NT, int W>
+// Long rows, any column range: the row's columns are swept in windows [lo, hi); a window starts
+// at the smallest next column over the row's A entries, so empty column ranges are skipped, and
+// every window's output is produced in column order (no sort). Each A entry is walked by a
+// subgroup of G lanes (long_lanes) from a cursor (LongState: position, remaining, next column).
+//
+// Rows whose entries each have a subgroup (nA <= NT/G) take windows of up to BW columns, with
+// two walks over the window's products:
+//   1. presence: every B column in the window sets its bit in an LDS bitmap; each A entry
+//      counts its elements in the window (U loads per lane in flight);
+//   2. values: the bitmap's word prefixes give every column its rank among the window's
+//      columns, and A_ik * B_kj is added at that rank in an LDS value array of VC entries.
+// The values are then stored as they lie (coalesced) and the columns read off the bitmap. When
+// more than VC columns are present the window is cut at the VC-th column (the values walk stops
+// there, the next window starts at it) and the next window is narrowed to the width that held
+// VC columns; windows widen again (doubling up to BW) when they fit. Between the two walks an
+// active entry keeps -1 - count in its `nc` field.
+//
+// Rows of more entries take dense windows of VC columns in one walk (values added at column -
+// lo, presence bits beside them), 4 entries per subgroup in flight; an entry whose next column
+// lies beyond the window costs one state read per window. Their cursors are in LDS (sharing the
+// word-prefix space) up to kOldStates entries, else in the workspace `state` (16 bytes per A
+// nonzero). Both paths leave the value array and the bitmap clear.
+constexpr int kNumLongStates = 512;
+
+template <typename V, int NT, int BW, int VC>
 __global__ __launch_bounds__(NT) void k_numeric_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
     const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
     const V* __restrict__ Aval, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
     const int64_t* __restrict__ Cptr, int64_t cbase, int32_t* __restrict__ Cidx, V* __restrict__ Cval,
     LongState* __restrict__ state, int* __restrict__ err) {
-  constexpr int NWORD = W / 32;
+  constexpr int NWORD = BW / 32;
+  constexpr int WPT = NWORD / NT;  // bitmap words per thread
+  constexpr int OWORD = VC / 32;   // bitmap words of a dense window
   constexpr int NW = NT / 64;
-  static_assert(NWORD <= NT && NT % 64 == 0, "one bitmap word per thread");
-  __shared__ V dv[W];
-  __shared__ uint32_t bm[NWORD];
-  __shared__ LongState stl[kLongStates];
-  __shared__ int red_min[NW], red_cnt[NW];
+  constexpr int U = 4;             // loads per lane in flight per walk step
+  constexpr int DVB = VC * (int)sizeof(V);
+  constexpr int kOldStates = (NWORD * 2 + kNumLongStates * 16) / 16;
+  static_assert(NWORD % NT == 0 && NT % 64 == 0 && VC < 65535 && OWORD <= NT, "shape");
+  __shared__ __align__(16) unsigned char lds[DVB + NWORD * 4 + NWORD * 2 + kNumLongStates * 16];
+  V* dv = reinterpret_cast<V*>(lds);
+  uint32_t* bm = reinterpret_cast<uint32_t*>(lds + DVB);
+  uint16_t* pre = reinterpret_cast<uint16_t*>(lds + DVB + NWORD * 4);  // exclusive prefix per word
+  LongState* stl = reinterpret_cast<LongState*>(lds + DVB + NWORD * 4 + NWORD * 2);
+  LongState* ostl = reinterpret_cast<LongState*>(lds + DVB + NWORD * 4);
+  __shared__ int red[NW], red_cnt[NW];
+  __shared__ long long s_hi2;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int q = t; q < W; q += NT) dv[q] = V(0);
   for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
+  for (int q = t; q < VC; q += NT) dv[q] = V(0);
   __syncthreads();
+  auto block_min = [&](int x) {  // all threads; ends with a barrier
+    const int m = wave_reduce_dpp(x, OpMin());
+    if (lane == 0) red[wv] = m;
+    __syncthreads();
+    int r = red[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r = min(r, red[i]);
+    __syncthreads();
+    return r;
+  };
   int flags = 0;
   const int64_t beg = *pbeg, end = *pend;
   for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
@@ -1452,27 +1484,10 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
     const int G = long_lanes<NT>(nA);
     const int NSG = NT / G, sg = t / G, lg = t & (G - 1), sgbase = lane & ~(G - 1);
     const uint64_t submask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << sgbase;
-    const bool regs = nA <= NSG;
-    LongState* st = nA <= kLongStates ? stl : state + a0;
-    int64_t rp = 0;
-    int rrem = 0, rnc = INT_MAX, pc = INT_MAX;
-    V rva = V(0), pv = V(0);
-    int lmin = INT_MAX;
-    if (regs) {
-      if (sg < nA) {
-        rp = bstart[a0 + sg];
-        rrem = blen[a0 + sg];
-        rva = Aval[a0 + sg];
-        const int64_t at = rrem > 0 ? rp + min(lg, rrem - 1) : 0;
-        const int cv = Bidx[at];
-        const V bv = Bval[at];
-        pc = lg < rrem ? cv : INT_MAX;
-        pv = bv;
-      }
-      rnc = __shfl(pc, sgbase);  // subgroup-uniform: all lanes of a subgroup share sg
-      if (sg >= nA) rnc = INT_MAX;
-      lmin = rnc;
-    } else {
+    if (nA <= NSG) {
+      // ================= two walks per window of up to BW columns =================
+      LongState* st = stl;
+      int lmin = INT_MAX;
       for (int64_t e = t; e < nA; e += NT) {
         LongState x;
         x.p = bstart[a0 + e];
@@ -1481,53 +1496,181 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
         st[e] = x;
         lmin = min(lmin, x.nc);
       }
-    }
-    {  // first window: smallest first column over the row
-      const int m = wave_reduce_dpp(lmin, OpMin());
-      if (lane == 0) red_min[wv] = m;
-      __syncthreads();
-      lmin = INT_MAX;
+      int lo = block_min(lmin);  // also orders the state writes before the walks
+      int64_t width = BW;
+      while (lo != INT_MAX) {
+        const int64_t hi1 = min((int64_t)lo + width, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
+        // ---- walk 1: presence bits of [lo, hi1), per-entry counts ----
+        for (int64_t e = sg; e < nA; e += NSG) {  // subgroup-uniform
+          LongState x = st[e];
+          if ((int64_t)x.nc < hi1) {
+            int64_t p = x.p;
+            int rem = x.rem, cnt = 0;
+            for (;;) {
+              int c[U];
 #pragma unroll
-      for (int i = 0; i < NW; ++i) lmin = min(lmin, red_min[i]);
-      __syncthreads();
-    }
-    int lo = lmin;
-    while (lo != INT_MAX) {
-      const int64_t hi = (int64_t)lo + W;
-      lmin = INT_MAX;
-      if (regs) {
-        if (rnc < hi) {  // subgroup-uniform
-          int c = pc;
-          V bv = pv;
-          for (;;) {
-            const bool in = c != INT_MAX && (int64_t)c < hi;  // INT_MAX: B row exhausted
-            if (in) {
-              const int o = c - lo;
-              lds_add(&dv[o], rva * bv);
-              atomicOr(&bm[o >> 5], 1u << (o & 31));
+              for (int u = 0; u < U; ++u) {
+                const int k = u * G + lg;
+                const int64_t at = rem > 0 ? p + min(k, rem - 1) : 0;
+                const int cv = Bidx[at];
+                c[u] = k < rem ? cv : INT_MAX;
+              }
+              int nin = 0;
+#pragma unroll
+              for (int u = 0; u < U; ++u) {
+                const bool in = (int64_t)c[u] < hi1;
+                if (in) {
+                  const int o = c[u] - lo;
+                  atomicOr(&bm[o >> 5], 1u << (o & 31));
+                }
+                nin += __popcll(__ballot(in) & submask);
+              }
+              cnt += nin;
+              if (nin < U * G) break;
+              p += U * G;
+              rem -= U * G;
             }
-            const int nin = __popcll(__ballot(in) & submask);
-            if (nin < G) {
-              rnc = __shfl(c, sgbase + nin);
-              rp += nin;
-              rrem -= nin;
-              break;
+            if (lg == 0) {
+              x.nc = -1 - cnt;
+              st[e] = x;
             }
-            rp += G;
-            rrem -= G;
-            const int64_t at = rrem > 0 ? rp + min(lg, rrem - 1) : 0;
-            const int cv = Bidx[at];
-            bv = Bval[at];
-            c = lg < rrem ? cv : INT_MAX;
           }
-          // the next window's first elements, loaded while this window is emitted
-          const int64_t at = rrem > 0 ? rp + min(lg, rrem - 1) : 0;
-          const int cv = Bidx[at];
-          pv = Bval[at];
-          pc = lg < rrem ? cv : INT_MAX;
         }
-        lmin = rnc;
-      } else {
+        __syncthreads();
+        // ---- word prefixes, and the cut when more than VC columns are present ----
+        uint32_t wd[WPT];
+        int wp[WPT];
+        int cnts = 0;
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+          wd[q] = bm[t * WPT + q];
+          cnts += __popc(wd[q]);
+        }
+        const int incl = wave_incl_scan_dpp(cnts);
+        if (lane == 63) red_cnt[wv] = incl;
+        __syncthreads();
+        int base = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+          const int ci = red_cnt[i];
+          base += i < wv ? ci : 0;
+          tot += ci;
+        }
+        base += incl - cnts;
+        {
+          int run = base;
+#pragma unroll
+          for (int q = 0; q < WPT; ++q) {
+            wp[q] = run;
+            pre[t * WPT + q] = (uint16_t)min(run, 65535);
+            run += __popc(wd[q]);
+          }
+        }
+        if (tot <= VC) {
+          if (t == 0) s_hi2 = hi1;
+        } else {
+#pragma unroll
+          for (int q = 0; q < WPT; ++q) {
+            const int pc = __popc(wd[q]);
+            if (wp[q] <= VC && VC < wp[q] + pc) {  // the VC-th set bit (from 0) is in this word
+              uint32_t w = wd[q];
+              for (int j = 0; j < VC - wp[q]; ++j) w &= w - 1u;
+              s_hi2 = (int64_t)lo + 32 * (t * WPT + q) + (__ffs(w) - 1);
+            }
+          }
+        }
+        __syncthreads();
+        const int64_t hi2 = s_hi2;
+        const int nw = min(tot, VC);
+        // ---- walk 2: values of [lo, hi2) at their ranks; cursors advance to hi2 ----
+        lmin = INT_MAX;
+        for (int64_t e = sg; e < nA; e += NSG) {  // subgroup-uniform
+          LongState x = st[e];
+          if (x.nc < 0) {
+            const int cnt = -1 - x.nc;
+            const int n = min(cnt + 1, x.rem);  // the window's elements and the one after it
+            const V va = Aval[a0 + e];
+            int nin2 = 0, nxt = INT_MAX;
+            for (int k0 = 0; k0 < n; k0 += U * G) {
+              int c[U];
+              V b[U];
+#pragma unroll
+              for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * G + lg;
+                const int64_t at = x.p + min(k, n - 1);
+                c[u] = Bidx[at];
+                b[u] = Bval[at];
+              }
+#pragma unroll
+              for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * G + lg;
+                const bool valid = k < n;
+                const bool in = valid && (int64_t)c[u] < hi2;
+                if (in) {
+                  const int o = c[u] - lo;
+                  const int w = o >> 5;
+                  const int r = (int)pre[w] + __popc(bm[w] & ((1u << (o & 31)) - 1u));
+                  lds_add(&dv[r], va * b[u]);
+                }
+                if (valid && !in) nxt = min(nxt, c[u]);
+                nin2 += __popcll(__ballot(in) & submask);
+              }
+            }
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+              if (o < G) nxt = min(nxt, __shfl_xor(nxt, o));
+            x.p += nin2;
+            x.rem -= nin2;
+            x.nc = x.rem > 0 ? nxt : INT_MAX;
+            if (lg == 0) st[e] = x;
+          }
+          lmin = min(lmin, x.nc);
+        }
+        __syncthreads();
+        // ---- emit: values as they lie, columns from the bitmap; both left clear ----
+        for (int i = t; i < nw; i += NT) {
+          if (out + i < c1) Cval[out + i] = dv[i];
+          else flags |= kErrWindow;
+          dv[i] = V(0);
+        }
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+          const int w = t * WPT + q;
+          const int64_t cb = (int64_t)lo + 32 * w;
+          uint32_t bits = wd[q];
+          if (cb >= hi2) bits = 0u;
+          else if (cb + 32 > hi2) bits &= (1u << (int)(hi2 - cb)) - 1u;
+          int64_t pos = out + wp[q];
+          while (bits) {
+            const int bb = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            if (pos < c1) Cidx[pos] = (int)(cb + bb);
+            else flags |= kErrWindow;
+            ++pos;
+          }
+          bm[w] = 0u;
+        }
+        out += nw;
+        // the width that held VC columns when cut, else twice the last width
+        width = tot > VC ? max((int64_t)64, hi2 - lo) : min((int64_t)BW, 2 * width);
+        lo = block_min(lmin);  // also: the clears are done before the next walk 1
+      }
+    } else {
+      // ================= dense windows of VC columns, one walk =================
+      LongState* st = nA <= kOldStates ? ostl : state + a0;
+      int lmin = INT_MAX;
+      for (int64_t e = t; e < nA; e += NT) {
+        LongState x;
+        x.p = bstart[a0 + e];
+        x.rem = blen[a0 + e];
+        x.nc = x.rem > 0 ? Bidx[x.p] : INT_MAX;
+        st[e] = x;
+        lmin = min(lmin, x.nc);
+      }
+      int lo = block_min(lmin);
+      while (lo != INT_MAX) {
+        const int64_t hi = (int64_t)lo + VC;
+        lmin = INT_MAX;
         // 4 entries per subgroup at a time: their first loads are issued together
         for (int64_t e0 = sg; e0 < nA; e0 += (int64_t)NSG * 4) {  // subgroup-uniform
           LongState x[4];
@@ -1583,44 +1726,44 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
             lmin = min(lmin, x[u].nc);
           }
         }
-      }
-      __syncthreads();  // the window's products are in LDS
-      // emit in column order: one bitmap word per thread, positions by a group scan
-      uint32_t w = t < NWORD ? bm[t] : 0u;
-      const int cnt = __popc(w);
-      const int incl = wave_incl_scan_dpp(cnt);
-      const int wm = wave_reduce_dpp(lmin, OpMin());
-      if (lane == 63) red_cnt[wv] = incl;
-      if (lane == 0) red_min[wv] = wm;
-      __syncthreads();
-      int base = 0, tot = 0, nlo = INT_MAX;
+        __syncthreads();  // the window's products are in LDS
+        // emit in column order: one bitmap word per thread, positions by a group scan
+        uint32_t w = t < OWORD ? bm[t] : 0u;
+        const int cnt = __popc(w);
+        const int incl = wave_incl_scan_dpp(cnt);
+        const int wm = wave_reduce_dpp(lmin, OpMin());
+        if (lane == 63) red_cnt[wv] = incl;
+        if (lane == 0) red[wv] = wm;
+        __syncthreads();
+        int base = 0, tot = 0, nlo = INT_MAX;
 #pragma unroll
-      for (int i = 0; i < NW; ++i) {
-        const int ci = red_cnt[i];
-        base += i < wv ? ci : 0;
-        tot += ci;
-        nlo = min(nlo, red_min[i]);
-      }
-      if (w) {
-        int64_t pos = out + base + incl - cnt;
-        bm[t] = 0u;
-        while (w) {
-          const int bb = __ffs(w) - 1;
-          w &= w - 1u;
-          const int o = t * 32 + bb;
-          if (pos < c1) {
-            Cidx[pos] = lo + o;
-            Cval[pos] = dv[o];
-          } else {
-            flags |= kErrWindow;
-          }
-          dv[o] = V(0);
-          ++pos;
+        for (int i = 0; i < NW; ++i) {
+          const int ci = red_cnt[i];
+          base += i < wv ? ci : 0;
+          tot += ci;
+          nlo = min(nlo, red[i]);
         }
+        if (w) {
+          int64_t pos = out + base + incl - cnt;
+          bm[t] = 0u;
+          while (w) {
+            const int bb = __ffs(w) - 1;
+            w &= w - 1u;
+            const int o = t * 32 + bb;
+            if (pos < c1) {
+              Cidx[pos] = lo + o;
+              Cval[pos] = dv[o];
+            } else {
+              flags |= kErrWindow;
+            }
+            dv[o] = V(0);
+            ++pos;
+          }
+        }
+        out += tot;
+        lo = nlo;
+        __syncthreads();  // cleared before the next window's products; reduction slots free
       }
-      out += tot;
-      lo = nlo;
-      __syncthreads();  // cleared before the next window's products; reduction slots free
     }
     if (out != c1) flags |= kErrWindow;
   }
@@ -1875,8 +2018,9 @@ static int grid_for(int64_t nrows, int gpb, int grid, const void* f = nullptr, i
 
 constexpr int kLongNT = 256;
 constexpr int kSymLongW = 131072;  // bits per symbolic window (16 KB bitmap)
-constexpr int kNumLongW64 = 4096;  // FP64 dense window (32 KB + 512 B bitmap; 3 workgroups per CU)
-constexpr int kNumLongW32 = 8192;  // FP32 dense window (32 KB + 1 KB bitmap)
+constexpr int kNumLongBW = 65536;   // columns per numeric window (8 KB bitmap + 4 KB word prefixes)
+constexpr int kNumLongVC64 = 4096;  // FP64 values per window (32 KB; with 8 KB of cursors: 3 workgroups per CU)
+constexpr int kNumLongVC32 = 8192;  // FP32 values per window (32 KB)
 constexpr int kBwdLongW64 = 8192;
 constexpr int kBwdLongW32 = 16384;
 constexpr int kLongG = 16;
@@ -1930,8 +2074,8 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
   const V* Bv = (const V*)B.val;
   if (k.T == 0) {
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
-    constexpr int W = sizeof(V) == 8 ? kNumLongW64 : kNumLongW32;
-    hipLaunchKernelGGL((k_numeric_long<V, kLongNT, W>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
+    constexpr int VC = sizeof(V) == 8 ? kNumLongVC64 : kNumLongVC32;
+    hipLaunchKernelGGL((k_numeric_long<V, kLongNT, kNumLongBW, VC>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
                        A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, (LongState*)cursor, err);
     return;
   }
