@@ -3,7 +3,8 @@
 //
 // Reference kernels replaced (S/ = dbd/for_small_datasets/src/ of the reference):
 //   calculate_UpperBound  S/preProc.cu:170-179   -> k_row_products (no atomics, int64)
-//   rowproduct_* + merge  S/computeGPU.cu:135-329 -> k_symbolic_hash / k_numeric_hash / *_long
+//   rowproduct_* + merge  S/computeGPU.cu:135-329 -> k_symbolic_hash, k_numeric_wave[_fast],
+//                                                   k_numeric_block, k_*_long
 //   collect               S/computeGPU.cu:331-341 -> not needed: exact offsets come from the
 //                                                   symbolic pass, numeric writes in place.
 // Differences on purpose: first touch is detected by a key compare-and-swap (the reference
