@@ -132,6 +132,34 @@ def test_long_rows_wide_column_range(gpu_ok, dtype):
     assert info["num_rows"][-1] == 50 and info["sym_rows"][-1] == 50, (info["num_rows"], info["sym_rows"])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_long_rows_dense_column_band(gpu_ok, dtype):
+    """Long rows whose output fills a dense band of 30,000 columns: the wide numeric windows
+    hold more columns than their value slots, so every window is cut at the slot count and the
+    next one narrowed (then widened again past the band)."""
+    rng = np.random.default_rng(5)
+    ncols = 100_000
+    rows_b, cols_b = [], []
+    for i in range(40):
+        c = np.concatenate([rng.choice(np.arange(10_000, 40_000), 3000, replace=False),
+                            rng.choice(np.concatenate([np.arange(0, 10_000), np.arange(40_000, ncols)]), 50,
+                                       replace=False)])
+        rows_b.append(np.full(c.size, i))
+        cols_b.append(c)
+    B = sp.csr_matrix((rng.standard_normal(sum(x.size for x in cols_b)),
+                       (np.concatenate(rows_b), np.concatenate(cols_b))), shape=(40, ncols))
+    rows_a, cols_a = [], []
+    for i in range(30):
+        c = rng.choice(40, size=8 + i, replace=False)
+        rows_a.append(np.full(c.size, i))
+        cols_a.append(c)
+    A = sp.csr_matrix((rng.standard_normal(sum(x.size for x in cols_a)),
+                       (np.concatenate(rows_a), np.concatenate(cols_a))), shape=(30, 40))
+    C, t, info = check_product(csr_sp(A, dtype), csr_sp(B, dtype), what="dense band", block_rows=512)
+    assert info["num_rows"][-1] == 30, info["num_rows"]
+    assert int(np.diff(C.ptr).min()) > 8192
+
+
 def test_block_rows_do_not_change_result(gpu_ok):
     A = csr_sp(_skewed_matrix(3000, 3000, {5: 900}, 21))
     ref = reference(A)
