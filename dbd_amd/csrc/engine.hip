@@ -111,7 +111,8 @@ struct Tables {
 };
 
 // Default table. Symbolic key: min(P_i, ncols) (a bound on nnz_i); numeric key: nnz_i.
-// Hash tables are sized so the load stays <= 1/2.
+// Hash tables are sized so the load stays <= `load`; the one-wave symbolic tables (keys only,
+// 4 bytes a slot, up to 1024 slots) to half of that, which shortens their probe sequences.
 Tables default_tables(int g_small, int g_big, double load) {
   Tables t;
   const int Ts[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
@@ -120,8 +121,8 @@ Tables default_tables(int g_small, int g_big, double load) {
   t.sym_ct.lim[n] = 0;
   t.sym_k[n++] = {0, 0, 0};
   for (int i = 0; i < 9; ++i) {
-    t.sym_ct.lim[n] = (int64_t)(Ts[i] * load);
-    t.sym_k[n++] = {Ts[i], NTs[i], Ts[i] <= 1024 ? g_small : g_big};
+    t.sym_ct.lim[n] = (int64_t)(Ts[i] * (Ts[i] <= 1024 ? load * 0.5 : load));
+    t.sym_k[n++] = {Ts[i], Ts[i] <= 1024 ? 64 : NTs[i], Ts[i] <= 1024 ? g_small : g_big};
   }
   t.sym_ct.lim[n] = INT64_MAX;
   t.sym_k[n++] = {0, 1024, 16};  // long rows

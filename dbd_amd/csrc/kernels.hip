@@ -2062,6 +2062,8 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
   DBD_SHAPES(X, 64)
   X(16384, 1024, 16)
   X(16384, 1024, 64)
+  X(1024, 64, 16)
+  X(1024, 64, 64)
 #undef X
 }
 
@@ -2163,7 +2165,7 @@ bool shape_available(int phase, const KernelShape& k) {
                     (T == 512 && NT == 64) || (T == 1024 && NT == 128) || (T == 2048 && NT == 256) ||
                     (T == 4096 && NT == 512) || (T == 8192 && NT == 1024);
   if (base) return true;
-  return phase == 0 && T == 16384 && NT == 1024;
+  return phase == 0 && ((T == 16384 && NT == 1024) || (T == 1024 && NT == 64));
 }
 
 }  // namespace dbd
