@@ -1450,6 +1450,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
   constexpr int OWORD = VC / 32;   // bitmap words of a dense window
   constexpr int NW = NT / 64;
   constexpr int U = 4;             // loads per lane in flight per walk step
+  constexpr int UL = 4;            // dense windows: steps of G elements loaded together on a long B row
   constexpr int DVB = VC * (int)sizeof(V);
   constexpr int kOldStates = (NWORD * 2 + kNumLongStates * 16) / 16;
   static_assert(NWORD % NT == 0 && NT % 64 == 0 && VC < 65535 && OWORD <= NT, "shape");
@@ -1699,13 +1700,15 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (x[u].nc < hi) {
-              int cu = c[u];
-              V bu = bv[u];
-              for (;;) {
+              // the first G elements were loaded above; a B row with more in the window (a long
+              // B row) continues UL steps of G at a time, their loads issued together
+              bool done = false;
+              {
+                const int cu = c[u];
                 const bool in = cu != INT_MAX && (int64_t)cu < hi;  // INT_MAX: B row exhausted
                 if (in) {
                   const int o = cu - lo;
-                  lds_add(&dv[o], va[u] * bu);
+                  lds_add(&dv[o], va[u] * bv[u]);
                   atomicOr(&bm[o >> 5], 1u << (o & 31));
                 }
                 const int nin = __popcll(__ballot(in) & submask);
@@ -1713,14 +1716,44 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
                   x[u].nc = __shfl(cu, sgbase + nin);
                   x[u].p += nin;
                   x[u].rem -= nin;
-                  break;
+                  done = true;
+                } else {
+                  x[u].p += G;
+                  x[u].rem -= G;
                 }
-                x[u].p += G;
-                x[u].rem -= G;
-                const int64_t at = x[u].rem > 0 ? x[u].p + min(lg, x[u].rem - 1) : 0;
-                const int cv = Bidx[at];
-                bu = Bval[at];
-                cu = lg < x[u].rem ? cv : INT_MAX;
+              }
+              while (!done) {  // subgroup-uniform
+                int cc[UL];
+                V bb[UL];
+#pragma unroll
+                for (int k = 0; k < UL; ++k) {
+                  const int kk = k * G + lg;
+                  const int64_t at = x[u].rem > 0 ? x[u].p + min(kk, x[u].rem - 1) : 0;
+                  const int cv = Bidx[at];
+                  bb[k] = Bval[at];
+                  cc[k] = kk < x[u].rem ? cv : INT_MAX;
+                }
+#pragma unroll
+                for (int k = 0; k < UL; ++k) {
+                  if (!done) {
+                    const bool in = cc[k] != INT_MAX && (int64_t)cc[k] < hi;
+                    if (in) {
+                      const int o = cc[k] - lo;
+                      lds_add(&dv[o], va[u] * bb[k]);
+                      atomicOr(&bm[o >> 5], 1u << (o & 31));
+                    }
+                    const int nin = __popcll(__ballot(in) & submask);
+                    if (nin < G) {
+                      x[u].nc = __shfl(cc[k], sgbase + nin);
+                      x[u].p += nin;
+                      x[u].rem -= nin;
+                      done = true;
+                    } else {
+                      x[u].p += G;
+                      x[u].rem -= G;
+                    }
+                  }
+                }
               }
               if (lg == 0) st[e0 + (int64_t)u * NSG] = x[u];
             }
