@@ -941,20 +941,23 @@ __device__ __forceinline__ int long_lanes(int64_t nA) {
 
 
 // Rows too long for an LDS table: sweep the row's columns in windows of W bits; one cursor
-// per A entry (cursor[] is indexed like A's nonzeros) walks its sorted B row.
+// per A entry (cursor[] is indexed like A's nonzeros) walks its sorted B row, whose extent
+// comes from the row analysis' descriptors. An entry with more elements in the window than one
+// step of G continues UL steps at a time, their loads issued together.
 template <int NT, int W, int G>
 __global__ __launch_bounds__(NT) void k_symbolic_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
-    const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx, const int64_t* __restrict__ Bptr,
+    const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
     const int32_t* __restrict__ Bidx, int64_t* __restrict__ nnz, int64_t* __restrict__ cursor,
     int* __restrict__ err) {
   constexpr int NWORD = W / 32;
   constexpr int WPT = NWORD / NT;
+  constexpr int UL = 4;
   static_assert(NWORD % NT == 0, "shape");
   __shared__ uint32_t bm[NWORD];
   __shared__ int sc[NT / 64 > 0 ? NT / 64 : 1];
   __shared__ long long scl[NT / 64 > 0 ? NT / 64 : 1];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lg = t % G;
   const int64_t beg = *pbeg, end = *pend;
   for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
     const int row = rows[slot];
@@ -963,33 +966,51 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
     for (int q = 0; q < WPT; ++q) bm[t + q * NT] = 0u;
     int lo_l = INT_MAX;
     for (int64_t ka = a0 + t; ka < a1; ka += NT) {
-      const int k = Aidx[ka];
-      const int64_t b0 = Bptr[k];
+      const int64_t b0 = bstart[ka];
       cursor[ka] = b0;
-      if (b0 < Bptr[k + 1]) lo_l = min(lo_l, Bidx[b0]);
+      if (blen[ka] > 0) lo_l = min(lo_l, Bidx[b0]);
     }
     int lo = Group<NT>::reduce(lo_l, OpMin(), sc);  // also orders the cursor writes
     long long count = 0;
     int guard = 0;
     while (lo != INT_MAX) {
-      const int64_t hi = (int64_t)lo + W;
+      const int64_t hi = min((int64_t)lo + W, (int64_t)INT_MAX);  // INT_MAX: past the B row
       for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
-        const int k = Aidx[ka];
         int64_t p = cursor[ka];
-        const int64_t b1 = Bptr[k + 1];
-        while (p < b1) {
-          const int64_t q = p + (t % G);
+        const int64_t b1 = bstart[ka] + blen[ka];
+        int nin;
+        {
+          const int64_t q = p + lg;
           const int c = (q < b1) ? Bidx[q] : INT_MAX;
           const bool in = (int64_t)c < hi;
           if (in) {
             const int o = c - lo;
             atomicOr(&bm[o >> 5], 1u << (o & 31));
           }
-          const int nin = __popcll(__ballot(in) & subgroup_mask<G>());
+          nin = __popcll(__ballot(in) & subgroup_mask<G>());
           p += nin;
-          if (nin < G) break;
         }
-        if ((t % G) == 0) cursor[ka] = p;
+        while (nin == G) {  // subgroup-uniform
+          int cc[UL];
+#pragma unroll
+          for (int k = 0; k < UL; ++k) {
+            const int64_t q = p + k * G + lg;
+            cc[k] = (q < b1) ? Bidx[q] : INT_MAX;
+          }
+#pragma unroll
+          for (int k = 0; k < UL; ++k) {
+            if (nin == G) {
+              const bool in = (int64_t)cc[k] < hi;
+              if (in) {
+                const int o = cc[k] - lo;
+                atomicOr(&bm[o >> 5], 1u << (o & 31));
+              }
+              nin = __popcll(__ballot(in) & subgroup_mask<G>());
+              p += nin;
+            }
+          }
+        }
+        if (lg == 0) cursor[ka] = p;
       }
       __syncthreads();
       long long pc = 0;
@@ -1001,9 +1022,8 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
       count += pc;
       int nl = INT_MAX;
       for (int64_t ka = a0 + t; ka < a1; ka += NT) {
-        const int k = Aidx[ka];
         const int64_t p = cursor[ka];
-        if (p < Bptr[k + 1]) nl = min(nl, Bidx[p]);
+        if (p < bstart[ka] + blen[ka]) nl = min(nl, Bidx[p]);
       }
       lo = Group<NT>::reduce(nl, OpMin(), sc);
       if (++guard > (1 << 30)) break;
@@ -1450,7 +1470,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
   constexpr int OWORD = VC / 32;   // bitmap words of a dense window
   constexpr int NW = NT / 64;
   constexpr int U = 4;             // loads per lane in flight per walk step
-  constexpr int UL = 4;            // dense windows: steps of G elements loaded together on a long B row
+  constexpr int UL = 8;            // dense windows: steps of G elements loaded together on a long B row
   constexpr int DVB = VC * (int)sizeof(V);
   constexpr int kOldStates = (NWORD * 2 + kNumLongStates * 16) / 16;
   static_assert(NWORD % NT == 0 && NT % 64 == 0 && VC < 65535 && OWORD <= NT, "shape");
@@ -1671,7 +1691,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
       }
       int lo = block_min(lmin);
       while (lo != INT_MAX) {
-        const int64_t hi = (int64_t)lo + VC;
+        const int64_t hi = min((int64_t)lo + VC, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
         lmin = INT_MAX;
         // 4 entries per subgroup at a time: their first loads are issued together
         for (int64_t e0 = sg; e0 < nA; e0 += (int64_t)NSG * 4) {  // subgroup-uniform
@@ -1949,7 +1969,7 @@ __global__ __launch_bounds__(NT) void k_backward_long(
     int bad = 0;
     while (cc < c1) {
       const int lo = Cidx[cc];
-      const int64_t hi = (int64_t)lo + W;
+      const int64_t hi = min((int64_t)lo + W, (int64_t)INT_MAX);  // INT_MAX: past the B row
       // C entries of this window: [cc, ce)
       // a window holds at most W distinct columns, so entries past cc + W are outside it
       long long ce_l = min((long long)c1, (long long)(cc + W));
@@ -2081,7 +2101,7 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
   if (k.T == 0) {
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     hipLaunchKernelGGL((k_symbolic_long<1024, kSymLongW, kLongG>), gr, dim3(1024), 0, s, rows, pbeg, pend, A.ptr,
-                       A.idx, B.ptr, B.idx, nnz, cursor, err);
+                       A.bstart, A.blen, B.idx, nnz, cursor, err);
     return;
   }
 #define X(T_, NT_, G_)                                                                                         \
