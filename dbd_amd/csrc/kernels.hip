@@ -1988,20 +1988,37 @@ __global__ __launch_bounds__(NT) void k_backward_long(
         int64_t p = cursor[ka];
         const int64_t b1 = Bptr[k + 1];
         V acc = V(0);
-        while (p < b1) {
-          const int64_t q = p + (t % G);
-          const int c = (q < b1) ? Bidx[q] : INT_MAX;
-          const bool in = (int64_t)c < hi;
-          if (in && c >= lo) {
-            const V gv = dv[c - lo];
-            acc += gv * Bval[q];
-            if (dB) atomicAdd(&dB[q], va * gv);
-          } else if (in) {
-            bad = 1;  // product column absent from C's row: C does not belong to these operands
+        // one step of G, then (a long B row) UL steps at a time with their loads issued together
+        int nin = G;
+        for (int first = 1; nin == G && p < b1; first = 0) {  // subgroup-uniform
+          constexpr int UL = 4;
+          int cc[UL];
+          V bb[UL];
+          const int ns = first ? 1 : UL;
+#pragma unroll
+          for (int k = 0; k < UL; ++k) {
+            const int64_t q = p + k * G + (t % G);
+            const bool ok = k < ns && q < b1;
+            cc[k] = ok ? Bidx[q] : INT_MAX;
+            bb[k] = ok ? Bval[q] : V(0);
           }
-          const int nin = __popcll(__ballot(in) & subgroup_mask<G>());
-          p += nin;
-          if (nin < G) break;
+#pragma unroll
+          for (int k = 0; k < UL; ++k) {
+            if (k < ns && nin == G) {
+              const int64_t q = p + (t % G);
+              const int c = cc[k];
+              const bool in = (int64_t)c < hi;
+              if (in && c >= lo) {
+                const V gv = dv[c - lo];
+                acc += gv * bb[k];
+                if (dB) atomicAdd(&dB[q], va * gv);
+              } else if (in) {
+                bad = 1;  // product column absent from C's row: C does not belong to these operands
+              }
+              nin = __popcll(__ballot(in) & subgroup_mask<G>());
+              p += nin;
+            }
+          }
         }
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
