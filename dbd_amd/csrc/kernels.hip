@@ -952,7 +952,7 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
     int* __restrict__ err) {
   constexpr int NWORD = W / 32;
   constexpr int WPT = NWORD / NT;
-  constexpr int UL = 4;
+  constexpr int UL = 8;
   static_assert(NWORD % NT == 0, "shape");
   __shared__ uint32_t bm[NWORD];
   __shared__ int sc[NT / 64 > 0 ? NT / 64 : 1];
