@@ -1452,10 +1452,11 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 // active entry keeps -1 - count in its `nc` field.
 //
 // Rows of more entries take dense windows of VC columns in one walk (values added at column -
-// lo, presence bits beside them), 4 entries per subgroup in flight; an entry whose next column
-// lies beyond the window costs one state read per window. Their cursors are in LDS (sharing the
-// word-prefix space) up to kOldStates entries, else in the workspace `state` (16 bytes per A
-// nonzero). Both paths leave the value array and the bitmap clear.
+// lo, presence bits beside them), 4 entries per subgroup in flight, a B row with more elements
+// in the window than one step continuing UL steps at a time (loads issued together); an entry
+// whose next column lies beyond the window costs one state read per window. Their cursors are in
+// LDS (sharing the word-prefix space) up to kOldStates entries, else in the workspace `state`
+// (16 bytes per A nonzero). Both paths leave the value array and the bitmap clear.
 constexpr int kNumLongStates = 512;
 
 template <typename V, int NT, int BW, int VC>
