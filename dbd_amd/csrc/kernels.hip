@@ -1452,7 +1452,8 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 // active entry keeps -1 - count in its `nc` field.
 //
 // Rows of more entries take dense windows of VC columns in one walk (values added at column -
-// lo, presence bits beside them), 4 entries per subgroup in flight, a B row with more elements
+// lo, presence bits beside them), 4 lanes per entry (8 when an entry averages 4 or more elements
+// per window over the row's column span), 4 entries per subgroup in flight, a B row with more elements
 // in the window than one step continuing UL steps at a time (loads issued together); an entry
 // whose next column lies beyond the window costs one state read per window. Their cursors are in
 // LDS (sharing the word-prefix space) up to kOldStates entries, else in the workspace `state`
@@ -1482,6 +1483,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
   LongState* stl = reinterpret_cast<LongState*>(lds + DVB + NWORD * 4 + NWORD * 2);
   LongState* ostl = reinterpret_cast<LongState*>(lds + DVB + NWORD * 4);
   __shared__ int red[NW], red_cnt[NW];
+  __shared__ long long redl[NW];
   __shared__ long long s_hi2;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
@@ -1504,9 +1506,9 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1], nA = a1 - a0;
     const int64_t c1 = Cptr[row + 1] - cbase;
     int64_t out = Cptr[row] - cbase;
-    const int G = long_lanes<NT>(nA);
-    const int NSG = NT / G, sg = t / G, lg = t & (G - 1), sgbase = lane & ~(G - 1);
-    const uint64_t submask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << sgbase;
+    int G = long_lanes<NT>(nA);
+    int NSG = NT / G, sg = t / G, lg = t & (G - 1), sgbase = lane & ~(G - 1);
+    uint64_t submask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << sgbase;
     if (nA <= NSG) {
       // ================= two walks per window of up to BW columns =================
       LongState* st = stl;
@@ -1681,16 +1683,33 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
     } else {
       // ================= dense windows of VC columns, one walk =================
       LongState* st = nA <= kOldStates ? ostl : state + a0;
-      int lmin = INT_MAX;
+      int lmin = INT_MAX, lmax = -1;
+      long long psum = 0;
       for (int64_t e = t; e < nA; e += NT) {
         LongState x;
         x.p = bstart[a0 + e];
         x.rem = blen[a0 + e];
         x.nc = x.rem > 0 ? Bidx[x.p] : INT_MAX;
+        if (x.rem > 0) lmax = max(lmax, Bidx[x.p + x.rem - 1]);
         st[e] = x;
         lmin = min(lmin, x.nc);
+        psum += x.rem;
       }
-      int lo = block_min(lmin);
+      // 8 lanes per entry when an entry averages 4 or more elements per window (its B row long
+      // next to the number of VC-column windows the row's column span takes), else 4
+      psum = Group<NT>::reduce(psum, OpAdd(), redl);
+      const int cmax = Group<NT>::reduce(lmax, OpMax(), red_cnt);
+      const int cmin = Group<NT>::reduce(lmin, OpMin(), red_cnt);
+      const double span = cmax >= cmin ? (double)cmax - (double)cmin + 1.0 : 1.0;
+      if ((double)psum * VC >= 4.0 * (double)nA * span) {
+        G = 8;
+        NSG = NT / G;
+        sg = t / G;
+        lg = t & (G - 1);
+        sgbase = lane & ~(G - 1);
+        submask = ((1ull << G) - 1ull) << sgbase;
+      }
+      int lo = cmin;  // the reductions above also ordered the state writes before the walk
       while (lo != INT_MAX) {
         const int64_t hi = min((int64_t)lo + VC, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
         lmin = INT_MAX;
