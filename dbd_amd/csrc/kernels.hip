@@ -1229,7 +1229,6 @@ __device__ __forceinline__ void numeric_hash_body(
   static_assert(NT % G == 0 && T % NT == 0, "shape");
   constexpr bool WAVE = (NT == 64);
   constexpr int NB = T / 2;  // buckets (workgroup path)
-  constexpr int PER = (NB + NT - 1) / NT;
   __shared__ int hk[GPB][T];
   __shared__ V hv[GPB][T];
   __shared__ int bc[GPB][WAVE ? 1 : NB + 1];
@@ -1911,7 +1910,6 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
     V* __restrict__ dA, V* __restrict__ dB, int* __restrict__ err) {
   constexpr int WG = wg_of<NT>();
   constexpr int GPB = WG / NT;
-  constexpr int NG = 64 / G;
   constexpr int U = 4;
   static_assert(NT % 64 == 0 && NT % G == 0 && T % NT == 0, "shape");
   __shared__ int hk[GPB][T];
@@ -1919,7 +1917,7 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_backward_hash(
   __shared__ int sc[GPB][NT > 64 ? NT / 64 : 1];
   const int t = threadIdx.x % NT;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);
-  const int lane = t & 63, wv = t >> 6, sg = lane / G, lg = lane % G;
+  const int lane = t & 63, wv = t >> 6;
   int* K = hk[g];
   V* H = hv[g];
   const int64_t beg = *pbeg, end = *pend;
