@@ -1,7 +1,7 @@
 """dbd_amd - MI355X-native Dynamic Block Distributor (CSR SpGEMM, C = A*B).
 
 Thin Python binding over the C ABI in include/dbd.h (libdbd.so, built in-tree by
-``python -m dbd_amd.build``). There is no Python or CPU fallback of the product: if the
+``python dbd_amd/build.py``). There is no Python or CPU fallback of the product: if the
 library is missing, importing the binding raises.
 
 Main entry points:

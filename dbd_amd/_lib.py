@@ -6,7 +6,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # DBD_LIBRARY selects another build of the same library (an A/B variant built with
-# `python -m dbd_amd.build --lib-out PATH -D NAME=VALUE`); default: the in-tree build.
+# `python dbd_amd/build.py --lib-out PATH -D NAME=VALUE`); default: the in-tree build.
 lib_path = os.environ.get("DBD_LIBRARY") or os.path.join(HERE, "libdbd.so")
 
 MODE_COMPUTE, MODE_D2H, MODE_IO = 0, 1, 2
@@ -58,7 +58,7 @@ class dbd_timing(C.Structure):
     _fields_ = [("preprocess_s", C.c_double), ("compute_gpu_s", C.c_double), ("write_s", C.c_double),
                 ("kernel_exec_s", C.c_double), ("total_s", C.c_double), ("h2d_s", C.c_double),
                 ("d2h_s", C.c_double), ("d2h_bytes", C.c_double), ("io_bytes", C.c_double),
-                ("chunks", C.c_int64)]
+                ("chunks", C.c_int64), ("symbolic_s", C.c_double), ("numeric_s", C.c_double)]
 
     def to_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,6 +80,7 @@ SIGNATURES = {
     "dbd_plan": (I32, [P, C.POINTER(dbd_plan_info)]),
     "dbd_run": (I32, [P, I32, C.POINTER(dbd_timing)]),
     "dbd_result": (I32, [P, CSRP]),
+    "dbd_copy_result": (I32, [P, P, P, P]),
     "dbd_get_plan_info": (I32, [P, C.POINTER(dbd_plan_info)]),
     "dbd_get_row_products": (I32, [P, P]),
     "dbd_get_blocks": (I32, [P, P, P]),
@@ -106,7 +107,7 @@ SIGNATURES = {
 
 def _load() -> C.CDLL:
     if not os.path.exists(lib_path):
-        raise ImportError(f"{lib_path} is not built; run `python -m dbd_amd.build` (hipcc, gfx950)")
+        raise ImportError(f"{lib_path} is not built; run `python dbd_amd/build.py` (hipcc, gfx950)")
     # One HIP runtime per process: torch ships its own libamdhip64 and asks for it by the
     # unversioned name, so if libdbd loaded the system copy first, torch would load a second
     # runtime and device pointers would not be shared. Loading torch first makes libdbd's
@@ -120,7 +121,7 @@ def _load() -> C.CDLL:
         f = getattr(lb, name)
         f.restype = res
         f.argtypes = args
-    if lb.dbd_abi_version() != 1:
+    if lb.dbd_abi_version() != 2:
         raise ImportError("libdbd ABI version mismatch")
     return lb
 

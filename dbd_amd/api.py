@@ -286,20 +286,15 @@ class Context:
         return v
 
     def result(self) -> CSR:
-        """Copy of the retained result as a host CSR (device results are copied via torch)."""
+        """Copy of the retained result (device or pinned host) as a host CSR; the library does
+        the copy on its own stream."""
         v = self.result_view()
         nr, nnz = v.nrows, v.nnz
-        npd = _np_dtype(v.dtype)
-        if v.location == HOST:
-            ptr = np.ctypeslib.as_array(C.cast(v.ptr, C.POINTER(C.c_int64)), shape=(nr + 1,)).copy()
-            idx = np.ctypeslib.as_array(C.cast(v.idx, C.POINTER(C.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
-            ct = C.c_double if v.dtype == F64 else C.c_float
-            val = np.ctypeslib.as_array(C.cast(v.val, C.POINTER(ct)), shape=(max(nnz, 1),))[:nnz].copy()
-            return CSR(ptr, idx, val, (nr, v.ncols))
-        from .devmem import copy_device_to_host
-        ptr = copy_device_to_host(v.ptr, (nr + 1) * 8, np.int64)
-        idx = copy_device_to_host(v.idx, nnz * 4, np.int32)
-        val = copy_device_to_host(v.val, nnz * np.dtype(npd).itemsize, npd)
+        ptr = np.empty(nr + 1, np.int64)
+        idx = np.empty(nnz, np.int32)
+        val = np.empty(nnz, _np_dtype(v.dtype))
+        check(lib.dbd_copy_result(self._h, ptr.ctypes.data, idx.ctypes.data if nnz else None,
+                                  val.ctypes.data if nnz else None))
         return CSR(ptr, idx, val, (nr, v.ncols))
 
     # -- device two-phase API (torch tensors) --
@@ -329,6 +324,7 @@ class Context:
         return int(ch.value)
 
     def batch_prepare(self, A_cat, B_cat) -> int:
+        """Plan the batch and run its symbolic pass once; returns the exact nnz of C_cat."""
         va, vb = A_cat.view(), B_cat.view()
         self._keep = [A_cat, B_cat, va, vb]
         n = C.c_int64()

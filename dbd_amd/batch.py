@@ -68,20 +68,27 @@ def split(Cptr: np.ndarray, Cidx: np.ndarray, Cval: np.ndarray, lay: BatchLayout
 
 
 class BatchRunner:
-    """Holds a context prepared for one batch; `launch()` replays the captured graph."""
+    """Holds a context prepared for one batch; `launch()` replays the captured graph. C is
+    allocated at the exact nnz that the preparation's symbolic pass reports. With tune=True the
+    category table is autotuned on the concatenation first (`table` holds the chosen id, for
+    running the plain per-matrix baseline with the same table)."""
 
-    def __init__(self, A_cat: CSR, B_cat: CSR, dtype=None, block_rows: int = 4096, tune: bool = False):
+    def __init__(self, A_cat: CSR, B_cat: CSR, dtype=None, block_rows: int = 4096, tune: bool = False,
+                 table: int | None = None):
         import torch
         self.dA, self.dB = DeviceCSR.from_host(A_cat), DeviceCSR.from_host(B_cat)
         self.ctx = Context(block_rows=block_rows, dtype=dtype or A_cat.val.dtype)
-        if tune:
+        self.table = table
+        if table is not None:
+            self.ctx.set_table(table)
+        elif tune:
             self.ctx.set_operands(self.dA, self.dB)
             self.table = self.ctx.autotune()
-        self.bound = self.ctx.batch_prepare(self.dA, self.dB)
+        self.nnz = self.ctx.batch_prepare(self.dA, self.dB)
         tdt = torch.float64 if A_cat.val.dtype == np.float64 else torch.float32
         self.Cptr = torch.empty(A_cat.shape[0] + 1, dtype=torch.int64, device="cuda")
-        self.Cidx = torch.empty(max(self.bound, 1), dtype=torch.int32, device="cuda")
-        self.Cval = torch.empty(max(self.bound, 1), dtype=tdt, device="cuda")
+        self.Cidx = torch.empty(max(self.nnz, 1), dtype=torch.int32, device="cuda")
+        self.Cval = torch.empty(max(self.nnz, 1), dtype=tdt, device="cuda")
 
     def launch(self):
         self.ctx.batch_launch(self.Cptr, self.Cidx, self.Cval)
@@ -95,12 +102,16 @@ class BatchRunner:
         self.ctx.close()
 
 
-def plain_each(As: list[CSR], Bs: list[CSR], block_rows: int = 4096, keep: bool = False):
+def plain_each(As: list[CSR], Bs: list[CSR], block_rows: int = 4096, keep: bool = False,
+               table: int | None = None):
     """Baseline: the plain pipeline once per member (plan, symbolic, numeric per matrix), with
-    operands already on the device. Returns the list of device results if keep."""
+    operands already on the device; `table` pins the category table (e.g. the batch's tuned
+    one). Returns the list of device results if keep."""
     import torch
     dev = [(DeviceCSR.from_host(A), DeviceCSR.from_host(B)) for A, B in zip(As, Bs)]
     ctx = Context(block_rows=block_rows, dtype=As[0].val.dtype)
+    if table is not None:
+        ctx.set_table(table)
     tdt = torch.float64 if As[0].val.dtype == np.float64 else torch.float32
 
     def run():

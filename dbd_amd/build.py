@@ -4,7 +4,7 @@ Outputs:
     dbd_amd/libdbd.so                 the C-ABI library (include/dbd.h)
     dbd_amd/bin/{compute,d2h,SSpMM,debug}   reference-compatible command lines
 
-Run: python -m dbd_amd.build   (or python dbd_amd/build.py)
+Run: python dbd_amd/build.py   (the package itself loads libdbd.so, so the build runs as a script)
 """
 from __future__ import annotations
 

@@ -112,6 +112,11 @@ int main(int argc, char** argv) {
   dbd_plan_info pi{};
   if (dbd_plan(ctx, &pi) != DBD_OK) return fail("plan");
   printf("Divide Matrix A into %lld Blocks (Block Size: %lld)\n\n", (long long)pi.num_blocks, (long long)pi.block_rows);
+#ifdef DBD_CLI_DEBUG
+  // block analysis before the run, as distribute_Workload prints it (Z/distributeWorkload.cpp:35-44)
+  printf("Top 10%% (Higher than %lld): %lld\n", (long long)pi.criteria, (long long)pi.blocks_gpu_only);
+  printf("Lower than Two Warp Size (64): %lld\n\n", (long long)pi.blocks_cpu_pref);
+#endif
   dbd_timing t{};
   if (dbd_run(ctx, DBD_CLI_MODE, &t) != DBD_OK) return fail("run");
   if (dbd_get_plan_info(ctx, &pi) != DBD_OK) return fail("plan info");
@@ -126,10 +131,14 @@ int main(int argc, char** argv) {
   printf("GPUs\tBlocks:   %lld (%.2f %%)\n\tElements: %lld (%.2f %%)\n\n", (long long)pi.num_blocks, 100.0,
          (long long)pi.products, 100.0);
 #ifdef DBD_CLI_DEBUG
+  // Z/main.cpp:110-116. Blocks reach the files one chunk per write (each chunk is a run of
+  // consecutive blocks written with one pwrite per file, the I/O-binding writer's coalescing,
+  // Z/writeFile_binding.cpp:60-168), so the blocks merged into another block's write are
+  // num_blocks - chunks.
   puts("\n - DEBUG -");
-  printf("Top 10%% (Higher than %lld): %lld\n", (long long)pi.criteria, (long long)pi.blocks_gpu_only);
-  printf("Lower than Two Warp Size (64): %lld\n\n", (long long)pi.blocks_cpu_pref);
-  printf("Output chunks: %lld\n", (long long)t.chunks);
+  const long long merged = t.chunks > 0 ? (long long)pi.num_blocks - (long long)t.chunks : 0;
+  printf("Merged %lld Blocks while Writing (%f)\n", merged,
+         pi.num_blocks > 0 ? (double)merged / (double)pi.num_blocks : 0.0);
   printf("Computed Total %lld Elements (%lld Bytes = %lld MebiBytes)\n", (long long)pi.products,
          (long long)pi.products * 4, (long long)(pi.products >> 18));
   printf("Actual Total %lld Elements (%lld Bytes = %lld MebiBytes)\n", (long long)pi.nnz, (long long)pi.nnz * 4,

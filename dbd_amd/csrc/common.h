@@ -17,6 +17,12 @@ namespace dbd {
 dbd_status set_error(dbd_status st, const char* fmt, ...);
 void clear_error();
 
+// Reference block criteria and categories (calculate_Criteria + analyze_Block) from per-block
+// product totals: fills num_blocks, block_rows, products, max_block_products, criteria and
+// blocks_{gpu_only,cpu_pref,any} of info. The one implementation the engine and
+// dbd_plan_blocks share.
+void block_criteria(const int64_t* block_products, int64_t nb, int32_t block_rows, dbd_plan_info* info);
+
 inline int hw_threads() {
   unsigned n = std::thread::hardware_concurrency();
   if (n == 0) n = 1;

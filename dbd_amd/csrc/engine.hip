@@ -286,7 +286,8 @@ struct dbd_ctx {
 
   // plan state
   Tables tb;
-  bool tuned_set = false;
+  bool tuned_set = false;       // a table was chosen for the current operands (tuned or explicit)
+  bool table_explicit = false;  // dbd_set_table was called: opts.tune does not override it
   bool planned = false, symb = false;
   int64_t nr = 0, nb = 0, nwg = 0;
   DevBuf P, Pptr, nnz, Cptr, rows_sym, rows_num, cnt, offs_sym, offs_num, tmp, cursor, blk, err, bdesc_s, bdesc_l;
@@ -306,6 +307,7 @@ struct dbd_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   const void* g_key[3] = {nullptr, nullptr, nullptr};
+  int64_t batch_cap = 0;  // entries of C the batch caller allocated (exact nnz at prepare)
   DevBuf g_rng;
 };
 
@@ -386,37 +388,7 @@ dbd_status plan(dbd_ctx* c) {
   for (int64_t b = 0; b < c->nb; ++b) c->h_blockP[(size_t)b] = c->h_blk[(size_t)b + 1] - c->h_blk[(size_t)b];
   dbd_plan_info& in = c->info;
   memset(&in, 0, sizeof(in));
-  std::vector<int64_t> bp((size_t)c->nb + 1);
-  {
-    // criteria from block totals: reuse the host logic on per-block sums
-    int64_t total = 0, mx = 0;
-    for (int64_t v : c->h_blockP) {
-      total += v;
-      mx = std::max(mx, v);
-    }
-    in.products = total;
-    in.max_block_products = mx;
-    in.num_blocks = c->nb;
-    in.block_rows = BR;
-    if (c->nb > 0) {
-      const int64_t avg = (total / c->nb) / BR;
-      float sd = 0.0f;
-      for (int64_t v : c->h_blockP) {
-        const int64_t d = avg - v / BR;
-        sd += (float)(d * d);
-      }
-      sd /= (float)c->nb;
-      sd = sqrtf(sd);
-      in.criteria = (int64_t)(1.28f * sd) + avg;
-      for (int64_t v : c->h_blockP) {
-        switch (dbd_analyze_block(in.criteria, v / BR)) {
-          case 0: ++in.blocks_gpu_only; break;
-          case 1: ++in.blocks_cpu_pref; break;
-          default: ++in.blocks_any; break;
-        }
-      }
-    }
-  }
+  block_criteria(c->h_blockP.data(), c->nb, BR, &in);
   in.nrows = nr;
   in.ncols = c->B.ncols;
   in.nnz = -1;
@@ -553,6 +525,7 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
   t.preprocess_s = t1 - t0;
   HIPCHK(hipEventRecord(c->ev_t0, s));
   CHK(symbolic(c));
+  HIPCHK(hipEventRecord(c->ev_c0, s));  // end of the symbolic pass (it synchronised the stream)
   const int64_t nnzC = c->info.nnz;
   const size_t esz = 4 + c->vsz;
   const int64_t budget = c->o.device_budget_mib << 20;
@@ -706,9 +679,12 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
     t.io_bytes = wr.bytes + (p64 ? 8.0 : 4.0) * (double)(c->nr + 1);
   }
   CHK(check_err(c, "numeric"));
-  float ms = 0;
+  float ms = 0, sym_ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1));
+  HIPCHK(hipEventElapsedTime(&sym_ms, c->ev_t0, c->ev_c0));
   comp_ms = ms;
+  t.symbolic_s = sym_ms * 1e-3;
+  t.numeric_s = (ms - sym_ms) * 1e-3;
   (void)copy_ms;
   const double t2 = now_s();
   t.compute_gpu_s = comp_ms * 1e-3;
@@ -876,7 +852,7 @@ dbd_status dbd_set_operands(dbd_ctx* c, const dbd_csr* A, const dbd_csr* B) {
                      (long long)A->nrows, (long long)A->ncols, (long long)B->nrows, (long long)B->ncols);
   c->ops = false;
   c->planned = c->symb = false;
-  c->tuned_set = false;
+  c->tuned_set = c->table_explicit;
   c->dev_result = c->host_result = false;
   const double t0 = now_s();
   CHK(upload(c, A, c->A, c->upA, c->a_bytes));
@@ -966,6 +942,18 @@ dbd_status dbd_result(dbd_ctx* c, dbd_csr* C) {
   return set_error(DBD_ERR_STATE, "no retained result (run COMPUTE or D2H with retain_result=1 within budget)");
 }
 
+dbd_status dbd_copy_result(dbd_ctx* c, int64_t* h_ptr, int32_t* h_idx, void* h_val) {
+  clear_error();
+  dbd_csr C;
+  CHK(dbd_result(c, &C));
+  const hipMemcpyKind k = C.location == DBD_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+  if (h_ptr) HIPCHK(hipMemcpyAsync(h_ptr, C.ptr, 8 * (size_t)(C.nrows + 1), k, c->st));
+  if (h_idx && C.nnz > 0) HIPCHK(hipMemcpyAsync(h_idx, C.idx, 4 * (size_t)C.nnz, k, c->st));
+  if (h_val && C.nnz > 0) HIPCHK(hipMemcpyAsync(h_val, C.val, c->vsz * (size_t)C.nnz, k, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return DBD_OK;
+}
+
 dbd_status dbd_symbolic(dbd_ctx* c, int64_t* d_Cptr, int64_t* nnz_out) {
   clear_error();
   if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
@@ -1036,6 +1024,10 @@ static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int3
   launch_bin_count(c->nnz.as<int64_t>(), nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s);
   launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_num.as<int64_t>(), (int64_t)nt.n * c->nwg, c->tmp.as<int64_t>(), s);
   launch_bin_scatter(c->nnz.as<int64_t>(), nr, INT64_MAX, nt, c->offs_num.as<int64_t>(), c->rows_num.as<int32_t>(), s);
+  // C was sized by the symbolic pass of dbd_batch_prepare; should the operands' structure have
+  // changed since, a larger output is refused on the device instead of overrunning C
+  launch_capacity_guard(d_Cptr + nr, c->batch_cap, c->offs_num.as<int64_t>(), (int64_t)nt.n * c->nwg + 1,
+                        c->err.as<int>(), s);
   const int64_t* no = c->offs_num.as<int64_t>();
   for (int k = 1; k < nt.n; ++k)
     launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), no + (int64_t)k * c->nwg,
@@ -1052,7 +1044,11 @@ dbd_status dbd_batch_prepare(dbd_ctx* c, const dbd_csr* A_cat, const dbd_csr* B_
   CHK(dbd_set_operands(c, A_cat, B_cat));
   CHK(plan(c));
   CHK(ensure_cursor(c));
-  if (nnz_bound) *nnz_bound = c->info.products;
+  // one eager symbolic pass: the exact size of C, which the caller allocates (the graph then
+  // recomputes the same counts on every replay)
+  CHK(symbolic(c));
+  c->batch_cap = c->info.nnz;
+  if (nnz_bound) *nnz_bound = c->info.nnz;
   return DBD_OK;
 }
 
@@ -1081,8 +1077,14 @@ dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* 
     memcpy(c->g_key, key, sizeof(key));
   }
   HIPCHK(hipGraphLaunch(c->gexec, c->st));
-  CHK(check_err(c, "batch"));
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   c->symb = false;
+  if (e & kErrCapacityBit)
+    return set_error(DBD_ERR_BUDGET, "batch: C needs more than the %lld entries dbd_batch_prepare reported "
+                     "(operand structure changed since); nothing was written", (long long)c->batch_cap);
+  if (e != 0) return set_error(DBD_ERR_INTERNAL, "batch: device consistency check failed (flags 0x%x)", e);
   return DBD_OK;
 }
 
@@ -1098,7 +1100,7 @@ dbd_status dbd_set_table(dbd_ctx* c, int32_t id) {
     return set_error(DBD_ERR_INVALID, "table id %d out of range [0, %d)", id, (int)candidate_tables().size());
   c->tb = candidate_tables()[(size_t)id];
   c->planned = c->symb = false;
-  c->tuned_set = true;  // an explicit choice is not overridden by opts.tune
+  c->tuned_set = c->table_explicit = true;  // an explicit choice is not overridden by opts.tune
   return DBD_OK;
 }
 
@@ -1108,14 +1110,17 @@ dbd_status dbd_autotune(dbd_ctx* c, int32_t* chosen) {
   CHK(ensure_events(c));
   CHK(plan(c));
   const std::string key = problem_class(c);
+  int hit = -1;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
-    if (it != g_tune_cache.end()) {
-      c->tb = candidate_tables()[(size_t)it->second];
-      if (chosen) *chosen = it->second;
-      return DBD_OK;
-    }
+    if (it != g_tune_cache.end()) hit = it->second;
+  }
+  if (hit >= 0) {  // re-plan: the bins and limits must be the chosen table's
+    c->tb = candidate_tables()[(size_t)hit];
+    CHK(plan(c));
+    if (chosen) *chosen = hit;
+    return DBD_OK;
   }
   // time symbolic + numeric for each candidate on the real operands
   int best = 0;

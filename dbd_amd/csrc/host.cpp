@@ -42,6 +42,52 @@ extern "C" {
 int32_t dbd_abi_version(void) { return DBD_ABI_VERSION; }
 const char* dbd_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+namespace dbd {
+
+// calculate_Criteria (Z/distributeWorkload.cpp:249-264) and analyze_Block per block. The
+// reference accumulates (avg - x)^2 in int (which overflows for large blocks) into a float;
+// here the square is formed in int64 and the float accumulation is kept.
+void block_criteria(const int64_t* block_products, int64_t nb, int32_t block_rows, dbd_plan_info* info) {
+  int64_t total = 0, mx = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    total += block_products[b];
+    mx = std::max(mx, block_products[b]);
+  }
+  info->num_blocks = nb;
+  info->block_rows = block_rows;
+  info->products = total;
+  info->max_block_products = mx;
+  int64_t criteria = 0, g = 0, c = 0, a = 0;
+  if (nb > 0) {
+    const int64_t avg = (total / nb) / block_rows;
+    float sd = 0.0f;
+    for (int64_t b = 0; b < nb; ++b) {
+      const int64_t d = avg - block_products[b] / block_rows;
+      sd += (float)(d * d);
+    }
+    sd /= (float)nb;
+    sd = sqrtf(sd);
+    criteria = (int64_t)(1.28f * sd) + avg;
+    for (int64_t b = 0; b < nb; ++b) {
+      switch (dbd_analyze_block(criteria, block_products[b] / block_rows)) {
+        case 0: ++g; break;
+        case 1: ++c; break;
+        default: ++a; break;
+      }
+    }
+  }
+  info->criteria = criteria;
+  info->blocks_gpu_only = g;
+  info->blocks_cpu_pref = c;
+  info->blocks_any = a;
+}
+
+}  // namespace dbd
+
+extern "C" {
+
 void dbd_default_opts(dbd_opts* o) {
   if (!o) return;
   memset(o, 0, sizeof(*o));
@@ -62,8 +108,6 @@ int32_t dbd_analyze_block(int64_t criteria, int64_t avg) {
 }
 
 // divide_Blocks (Z/main.cpp:259-296) + calculate_Criteria (Z/distributeWorkload.cpp:249-264).
-// The reference accumulates (avg - x)^2 in int (which overflows for large blocks) into a float;
-// here the square is formed in int64 and the float accumulation is kept.
 dbd_status dbd_plan_blocks(const int64_t* P, int64_t nr, int32_t block_rows, int64_t* block_ptr,
                            int64_t* block_products, dbd_plan_info* info) {
   if (nr < 0 || block_rows <= 0 || (nr > 0 && (!P || !block_ptr || !block_products)))
@@ -82,33 +126,7 @@ dbd_status dbd_plan_blocks(const int64_t* P, int64_t nr, int32_t block_rows, int
   if (nb > 0) block_ptr[nb] = nr;
   if (info) {
     info->nrows = nr;
-    info->num_blocks = nb;
-    info->block_rows = block_rows;
-    info->products = total;
-    info->max_block_products = mx;
-    int64_t criteria = 0, g = 0, c = 0, a = 0;
-    if (nb > 0) {
-      const int64_t avg = (total / nb) / block_rows;
-      float sd = 0.0f;
-      for (int64_t b = 0; b < nb; ++b) {
-        const int64_t d = avg - block_products[b] / block_rows;
-        sd += (float)(d * d);
-      }
-      sd /= (float)nb;
-      sd = sqrtf(sd);
-      criteria = (int64_t)(1.28f * sd) + avg;
-      for (int64_t b = 0; b < nb; ++b) {
-        switch (dbd_analyze_block(criteria, block_products[b] / block_rows)) {
-          case 0: ++g; break;
-          case 1: ++c; break;
-          default: ++a; break;
-        }
-      }
-    }
-    info->criteria = criteria;
-    info->blocks_gpu_only = g;
-    info->blocks_cpu_pref = c;
-    info->blocks_any = a;
+    block_criteria(block_products, nb, block_rows, info);
   }
   return DBD_OK;
 }
@@ -384,8 +402,10 @@ dbd_status dbd_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c,
 
 // ---- file formats ---------------------------------------------------------------------
 
-dbd_status dbd_write_csr_cache(const char* path, const dbd_csr* M) {
-  clear_error();
+// The reference's cache (Z/readInput.cpp:166-175) holds FP32 values. FP64 reads keep their own
+// cache, `<file>.csr64`: the same int32 header, row pointer and columns, then FP64 values, so an
+// FP64 run never multiplies values that went through float.
+static dbd_status write_cache(const char* path, const dbd_csr* M, bool f64) {
   if (!path || !M || M->location != DBD_HOST) return set_error(DBD_ERR_INVALID, "dbd_write_csr_cache: bad arguments");
   if (M->nrows >= INT32_MAX || M->ncols > INT32_MAX || M->nnz > INT32_MAX)
     return set_error(DBD_ERR_OVERFLOW, "the .csr cache format is int32; matrix too large");
@@ -397,15 +417,26 @@ dbd_status dbd_write_csr_cache(const char* path, const dbd_csr* M) {
   for (int64_t i = 0; i <= M->nrows; ++i) p32[(size_t)i] = (int32_t)M->ptr[i];
   ok = ok && fwrite(p32.data(), 4, p32.size(), fp) == p32.size();
   ok = ok && fwrite(M->idx, 4, (size_t)M->nnz, fp) == (size_t)M->nnz;
-  std::vector<float> v32((size_t)M->nnz);
-  for (int64_t i = 0; i < M->nnz; ++i) v32[(size_t)i] = (float)load_val(M->val, M->dtype, i);
-  ok = ok && fwrite(v32.data(), 4, v32.size(), fp) == v32.size();
+  if (f64) {
+    std::vector<double> v((size_t)M->nnz);
+    for (int64_t i = 0; i < M->nnz; ++i) v[(size_t)i] = load_val(M->val, M->dtype, i);
+    ok = ok && fwrite(v.data(), 8, v.size(), fp) == v.size();
+  } else {
+    std::vector<float> v32((size_t)M->nnz);
+    for (int64_t i = 0; i < M->nnz; ++i) v32[(size_t)i] = (float)load_val(M->val, M->dtype, i);
+    ok = ok && fwrite(v32.data(), 4, v32.size(), fp) == v32.size();
+  }
   ok = (fclose(fp) == 0) && ok;
   if (!ok) return set_error(DBD_ERR_IO, "write to %s failed", path);
   return DBD_OK;
 }
 
-static dbd_status read_cache(const char* path, int32_t dtype, dbd_csr* out) {
+dbd_status dbd_write_csr_cache(const char* path, const dbd_csr* M) {
+  clear_error();
+  return write_cache(path, M, false);
+}
+
+static dbd_status read_cache(const char* path, int32_t dtype, bool f64, dbd_csr* out) {
   FILE* fp = fopen(path, "rb");
   if (!fp) return set_error(DBD_ERR_IO, "cannot open %s", path);
   int32_t hdr[3];
@@ -419,17 +450,19 @@ static dbd_status read_cache(const char* path, int32_t dtype, dbd_csr* out) {
     return st;
   }
   std::vector<int32_t> p32((size_t)hdr[0] + 1);
-  std::vector<float> v32((size_t)hdr[2]);
+  std::vector<float> v32(f64 ? 0 : (size_t)hdr[2]);
+  std::vector<double> v64(f64 ? (size_t)hdr[2] : 0);
   bool ok = fread(p32.data(), 4, p32.size(), fp) == p32.size();
   ok = ok && fread(out->idx, 4, (size_t)hdr[2], fp) == (size_t)hdr[2];
-  ok = ok && fread(v32.data(), 4, v32.size(), fp) == v32.size();
+  ok = ok && (f64 ? fread(v64.data(), 8, v64.size(), fp) == v64.size()
+                  : fread(v32.data(), 4, v32.size(), fp) == v32.size());
   fclose(fp);
   if (!ok) {
     dbd_csr_free(out);
     return set_error(DBD_ERR_INVALID, "%s: truncated .csr file", path);
   }
   for (size_t i = 0; i < p32.size(); ++i) out->ptr[i] = p32[i];
-  for (int64_t i = 0; i < hdr[2]; ++i) store_val(out->val, dtype, i, v32[(size_t)i]);
+  for (int64_t i = 0; i < hdr[2]; ++i) store_val(out->val, dtype, i, f64 ? v64[(size_t)i] : (double)v32[(size_t)i]);
   if (out->ptr[0] != 0 || out->ptr[hdr[0]] != hdr[2]) {
     dbd_csr_free(out);
     return set_error(DBD_ERR_INVALID, "%s: inconsistent .csr row pointer", path);
@@ -447,10 +480,9 @@ dbd_status dbd_read_matrix(const char* path, int32_t dtype, int32_t use_cache, d
   clear_error();
   if (!path || !out) return set_error(DBD_ERR_INVALID, "dbd_read_matrix: bad arguments");
   std::string p(path);
-  if (use_cache) {
-    std::string cp = p + ".csr";
-    if (access(cp.c_str(), F_OK) == 0) return read_cache(cp.c_str(), dtype, out);
-  }
+  const bool f64 = dtype == DBD_F64;
+  const std::string cache_path = p + (f64 ? ".csr64" : ".csr");
+  if (use_cache && access(cache_path.c_str(), F_OK) == 0) return read_cache(cache_path.c_str(), dtype, f64, out);
   FILE* fp = fopen(path, "rt");
   if (!fp) return set_error(DBD_ERR_IO, "cannot open %s: %s", path, strerror(errno));
   char line[4096];
@@ -532,8 +564,7 @@ dbd_status dbd_read_matrix(const char* path, int32_t dtype, int32_t use_cache, d
     store_val(out->val, dtype, (int64_t)i, es[i].v);
   }
   if (use_cache && out->nnz <= INT32_MAX) {
-    std::string cp = p + ".csr";
-    dbd_write_csr_cache(cp.c_str(), out);  // best effort, like the reference (:166-175)
+    write_cache(cache_path.c_str(), out, f64);  // best effort, like the reference (:166-175)
     clear_error();
   }
   return DBD_OK;

@@ -47,6 +47,11 @@ void launch_exclusive_scan(const int64_t* in, int64_t* out, int64_t n, int64_t* 
 // out[b] = ptr[min(b*stride, n)] for b in [0, nb]
 void launch_gather_strided(const int64_t* ptr, int64_t n, int64_t stride, int64_t nb, int64_t* out, hipStream_t s);
 
+// Inside a captured pipeline: if *total > cap, set the capacity bit (16) in *err and zero
+// offs[0 .. n), which empties every bin range that the following row kernels read.
+constexpr int kErrCapacityBit = 16;
+void launch_capacity_guard(const int64_t* total, int64_t cap, int64_t* offs, int64_t n, int* err, hipStream_t s);
+
 // ---- planning ----
 // P[i] = sum_{k in A_i} len(B_k); nnz[i] = 0 where P[i] == 0; per A nonzero j with column k:
 // bstart[j] = B.ptr[k], blen[j] = len(B_k) (both nullable).

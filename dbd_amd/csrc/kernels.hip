@@ -622,6 +622,7 @@ constexpr int kErrTableFull = 1;
 constexpr int kErrCountMismatch = 2;
 constexpr int kErrWindow = 4;
 constexpr int kErrMissingKey = 8;
+constexpr int kErrCapacity = 16;  // output larger than the capacity the caller provided
 
 // ------------------------------------------------------------------------------------
 // fills, scan, gathers
@@ -734,6 +735,21 @@ __global__ void k_gather_strided(const int64_t* __restrict__ ptr, int64_t n, int
 
 void launch_gather_strided(const int64_t* ptr, int64_t n, int64_t stride, int64_t nb, int64_t* out, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_strided, dim3(fill_grid(nb + 1)), dim3(256), 0, s, ptr, n, stride, nb, out);
+}
+
+// Output capacity check inside a captured pipeline: when the exact output size (*total) exceeds
+// the caller's capacity, flag it and empty every bin range (offs[0 .. n)), so the kernels that
+// follow drain nothing and write nothing.
+__global__ void k_capacity_guard(const int64_t* __restrict__ total, int64_t cap, int64_t* __restrict__ offs, int64_t n,
+                                 int* __restrict__ err) {
+  if (*total <= cap) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrCapacity);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    offs[i] = 0;
+}
+
+void launch_capacity_guard(const int64_t* total, int64_t cap, int64_t* offs, int64_t n, int* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_capacity_guard, dim3(fill_grid(n)), dim3(256), 0, s, total, cap, offs, n, err);
 }
 
 // ------------------------------------------------------------------------------------
@@ -965,14 +981,19 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 #pragma unroll
     for (int q = 0; q < WPT; ++q) bm[t + q * NT] = 0u;
     int lo_l = INT_MAX;
+    long long prow_l = 0;
     for (int64_t ka = a0 + t; ka < a1; ka += NT) {
       const int64_t b0 = bstart[ka];
       cursor[ka] = b0;
+      prow_l += blen[ka];
       if (blen[ka] > 0) lo_l = min(lo_l, Bidx[b0]);
     }
+    // every window takes at least the element at its first column, so a row of P_i products
+    // needs at most P_i windows; more would mean a walk that stopped making progress
+    const long long prow = Group<NT>::reduce(prow_l, OpAdd(), scl);
     int lo = Group<NT>::reduce(lo_l, OpMin(), sc);  // also orders the cursor writes
     long long count = 0;
-    int guard = 0;
+    long long windows = 0;
     while (lo != INT_MAX) {
       const int64_t hi = min((int64_t)lo + W, (int64_t)INT_MAX);  // INT_MAX: past the B row
       for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
@@ -990,7 +1011,10 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
           nin = __popcll(__ballot(in) & subgroup_mask<G>());
           p += nin;
         }
-        while (nin == G) {  // subgroup-uniform
+        // B rows with more elements in the window than one step: UL steps at a time. p grows by
+        // G per full step and never passes b1 on this path, so the walk ends within len / G steps
+        // whatever the sentinel column past the row compares as.
+        while (nin == G && p < b1) {  // subgroup-uniform
           int cc[UL];
 #pragma unroll
           for (int k = 0; k < UL; ++k) {
@@ -1026,7 +1050,10 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
         if (p < bstart[ka] + blen[ka]) nl = min(nl, Bidx[p]);
       }
       lo = Group<NT>::reduce(nl, OpMin(), sc);
-      if (++guard > (1 << 30)) break;
+      if (++windows > prow) {  // uniform: no progress; reported, the row's count is not trusted
+        if (t == 0) atomicOr(err, kErrWindow);
+        break;
+      }
     }
     count = Group<NT>::reduce(count, OpAdd(), scl);
     if (t == 0) nnz[row] = count;
@@ -1512,6 +1539,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
       // ================= two walks per window of up to BW columns =================
       LongState* st = stl;
       int lmin = INT_MAX;
+      long long psum = 0;
       for (int64_t e = t; e < nA; e += NT) {
         LongState x;
         x.p = bstart[a0 + e];
@@ -1519,9 +1547,13 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
         x.nc = x.rem > 0 ? Bidx[x.p] : INT_MAX;
         st[e] = x;
         lmin = min(lmin, x.nc);
+        psum += x.rem;
       }
+      // a window always takes the element at its first column: at most P_i windows per row
+      const long long prow = Group<NT>::reduce(psum, OpAdd(), redl);
       int lo = block_min(lmin);  // also orders the state writes before the walks
       int64_t width = BW;
+      long long windows = 0;
       while (lo != INT_MAX) {
         const int64_t hi1 = min((int64_t)lo + width, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
         // ---- walk 1: presence bits of [lo, hi1), per-entry counts ----
@@ -1550,7 +1582,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
                 nin += __popcll(__ballot(in) & submask);
               }
               cnt += nin;
-              if (nin < U * G) break;
+              if (nin < U * G || rem <= U * G) break;  // rem shrinks every step: bounded walk
               p += U * G;
               rem -= U * G;
             }
@@ -1678,6 +1710,10 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
         // the width that held VC columns when cut, else twice the last width
         width = tot > VC ? max((int64_t)64, hi2 - lo) : min((int64_t)BW, 2 * width);
         lo = block_min(lmin);  // also: the clears are done before the next walk 1
+        if (++windows > prow) {  // uniform: no progress (reported; the row's output is not trusted)
+          flags |= kErrWindow;
+          break;
+        }
       }
     } else {
       // ================= dense windows of VC columns, one walk =================
@@ -1709,6 +1745,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
         submask = ((1ull << G) - 1ull) << sgbase;
       }
       int lo = cmin;  // the reductions above also ordered the state writes before the walk
+      long long windows = 0;  // at most psum windows (each takes the element at its first column)
       while (lo != INT_MAX) {
         const int64_t hi = min((int64_t)lo + VC, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
         lmin = INT_MAX;
@@ -1761,6 +1798,9 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
                   x[u].rem -= G;
                 }
               }
+              // bounded: every pass either ends (fewer than G in the window) or takes G elements
+              // off rem, and with rem <= 0 every lane holds the INT_MAX sentinel, which is
+              // excluded explicitly, so the pass ends
               while (!done) {  // subgroup-uniform
                 int cc[UL];
                 V bb[UL];
@@ -1836,6 +1876,10 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
         out += tot;
         lo = nlo;
         __syncthreads();  // cleared before the next window's products; reduction slots free
+        if (++windows > psum) {  // uniform: no progress (reported; the row's output is not trusted)
+          flags |= kErrWindow;
+          break;
+        }
       }
     }
     if (out != c1) flags |= kErrWindow;

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define DBD_ABI_VERSION 1
+#define DBD_ABI_VERSION 2
 
 typedef enum dbd_status {
   DBD_OK = 0,
@@ -115,6 +115,8 @@ typedef struct dbd_timing {
   double d2h_bytes;     /* bytes copied D2H */
   double io_bytes;      /* bytes written to files */
   int64_t chunks;       /* output chunks used */
+  double symbolic_s;    /* device time of the symbolic pass (row counts, scan, numeric bins) */
+  double numeric_s;     /* device time from the end of the symbolic pass to the last numeric chunk */
 } dbd_timing;
 
 typedef struct dbd_ctx dbd_ctx;
@@ -139,6 +141,10 @@ dbd_status dbd_run(dbd_ctx* ctx, int32_t mode, dbd_timing* timing);
 /* Retained result: device CSR after COMPUTE (retain_result=1), host CSR after D2H.
  * The arrays stay owned by ctx until the next run or dbd_destroy. */
 dbd_status dbd_result(dbd_ctx* ctx, dbd_csr* C);
+/* Copy the retained result into caller host arrays (nrows+1 int64, nnz int32, nnz values; any
+ * may be NULL to skip). Counterpart of the reference's result buffers being read by its writer
+ * (Z/writeFile.cpp:58-154); here for callers that want C in their own memory. */
+dbd_status dbd_copy_result(dbd_ctx* ctx, int64_t* host_ptr, int32_t* host_idx, void* host_val);
 dbd_status dbd_get_plan_info(dbd_ctx* ctx, dbd_plan_info* info);
 /* Per-row product counts P_i (host array of nrows) and block table (num_blocks + 1 entries
  * of block_ptr and num_blocks of block_products) from the last plan. */
@@ -161,7 +167,12 @@ dbd_status dbd_backward(dbd_ctx* ctx, const int64_t* d_Cptr, const int32_t* d_Ci
  * indices already offset into B_cat's rows; B_cat column indices offset per matrix), so one
  * plan covers every member. Persistent category kernels read their work ranges from device
  * memory, so the whole pipeline is captured once and replayed with one hipGraphLaunch. */
-dbd_status dbd_batch_prepare(dbd_ctx* ctx, const dbd_csr* A_cat, const dbd_csr* B_cat, int64_t* nnz_bound);
+/* dbd_batch_prepare plans the batch and runs its symbolic pass once: *nnz_out is the exact
+ * nnz(C_cat), the number of column indices and values the caller allocates. dbd_batch_launch
+ * replays the captured pipeline into Cptr (nrows+1), Cidx and Cval (nnz_out entries each); if the
+ * operands' structure changed so that C would need more, nothing is written and DBD_ERR_BUDGET
+ * is returned. */
+dbd_status dbd_batch_prepare(dbd_ctx* ctx, const dbd_csr* A_cat, const dbd_csr* B_cat, int64_t* nnz_out);
 dbd_status dbd_batch_launch(dbd_ctx* ctx, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval);
 /* Autotuner: time candidate category tables on the operands set in ctx and keep the fastest
  * for this problem class (cached in-process). Returns the chosen table's id. */

@@ -38,7 +38,7 @@ def _load():
     lb.oracle_spgemm_count.restype = I64
     lb.oracle_spgemm_fill.argtypes = [I64, I64, P, P, P, P, P, P, P, P, P, P, C.c_int]
     lb.oracle_spgemm_fill.restype = None
-    lb.oracle_spgemm_rows.argtypes = [I64, P, I64, P, P, P, P, P, P, P, P, P, C.c_int]
+    lb.oracle_spgemm_rows.argtypes = [I64, P, I64, P, P, P, P, P, P, P, P, P, P, C.c_int]
     lb.oracle_spgemm_rows.restype = I64
     lb.oracle_backward.argtypes = [I64, P, P, P, P, P, P, P, P, P, P, P, I64]
     lb.oracle_backward.restype = I64
@@ -112,8 +112,8 @@ def spgemm(A, B=None, threads: int = 0, with_abs: bool = False):
     return Cptr, Cidx[:nnz], Cval[:nnz]
 
 
-def spgemm_rows(rows, A, B=None, threads: int = 0):
-    """Rows `rows` of A*B only: returns (Sptr, Sidx, Sval) over the selected rows."""
+def spgemm_rows(rows, A, B=None, threads: int = 0, with_abs: bool = False):
+    """Rows `rows` of A*B only: returns (Sptr, Sidx, Sval[, Sabs]) over the selected rows."""
     Bm = B if B is not None else A
     sel = _c(rows, np.int32)
     Aptr, Aidx, Aval = _c(A.ptr, np.int64), _c(A.idx, np.int32), _c(A.val, np.float64)
@@ -122,12 +122,15 @@ def spgemm_rows(rows, A, B=None, threads: int = 0):
     L = lib()
     nnz = L.oracle_spgemm_rows(sel.shape[0], sel.ctypes.data, Bm.shape[1], Aptr.ctypes.data, Aidx.ctypes.data,
                                Aval.ctypes.data, Bptr.ctypes.data, Bidx.ctypes.data, Bval.ctypes.data,
-                               Sptr.ctypes.data, None, None, threads)
+                               Sptr.ctypes.data, None, None, None, threads)
     Sidx = np.zeros(max(nnz, 1), np.int32)
     Sval = np.zeros(max(nnz, 1), np.float64)
+    Sabs = np.zeros(max(nnz, 1), np.float64) if with_abs else None
     L.oracle_spgemm_rows(sel.shape[0], sel.ctypes.data, Bm.shape[1], Aptr.ctypes.data, Aidx.ctypes.data,
                          Aval.ctypes.data, Bptr.ctypes.data, Bidx.ctypes.data, Bval.ctypes.data, Sptr.ctypes.data,
-                         Sidx.ctypes.data, Sval.ctypes.data, threads)
+                         Sidx.ctypes.data, Sval.ctypes.data, Sabs.ctypes.data if with_abs else None, threads)
+    if with_abs:
+        return Sptr, Sidx[:nnz], Sval[:nnz], Sabs[:nnz]
     return Sptr, Sidx[:nnz], Sval[:nnz]
 
 

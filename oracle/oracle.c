@@ -194,11 +194,11 @@ void oracle_spgemm_fill(int64_t nr, int64_t ncols, const int64_t* Aptr, const in
 }
 
 /* Rows `sel[0..ns)` only (for sampled checks of very large products). Output CSR over the
- * selected rows: Sptr (ns + 1), Sidx / Sval sized by the caller from the returned count of a
- * first call with Sidx == NULL. */
+ * selected rows: Sptr (ns + 1), Sidx / Sval (and optionally Sabs, as in oracle_spgemm_fill)
+ * sized by the caller from the returned count of a first call with Sidx == NULL. */
 int64_t oracle_spgemm_rows(int64_t ns, const int32_t* sel, int64_t ncols, const int64_t* Aptr, const int32_t* Aidx,
                            const double* Aval, const int64_t* Bptr, const int32_t* Bidx, const double* Bval,
-                           int64_t* Sptr, int32_t* Sidx, double* Sval, int threads) {
+                           int64_t* Sptr, int32_t* Sidx, double* Sval, double* Sabs, int threads) {
   int64_t* Aptr_s = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ns + 1));
   int64_t tot = 0;
   Aptr_s[0] = 0;
@@ -213,7 +213,7 @@ int64_t oracle_spgemm_rows(int64_t ns, const int32_t* sel, int64_t ncols, const 
     if (Aval) memcpy(Aval_s + Aptr_s[r], Aval + Aptr[sel[r]], sizeof(double) * (size_t)(Aptr_s[r + 1] - Aptr_s[r]));
   }
   const int64_t nnz = oracle_spgemm_count(ns, ncols, Aptr_s, Aidx_s, Bptr, Bidx, Sptr, threads);
-  if (Sidx && Sval) oracle_spgemm_fill(ns, ncols, Aptr_s, Aidx_s, Aval_s, Bptr, Bidx, Bval, Sptr, Sidx, Sval, NULL, threads);
+  if (Sidx && Sval) oracle_spgemm_fill(ns, ncols, Aptr_s, Aidx_s, Aval_s, Bptr, Bidx, Bval, Sptr, Sidx, Sval, Sabs, threads);
   free(Aptr_s);
   free(Aidx_s);
   free(Aval_s);

@@ -22,19 +22,6 @@ def make_batch(n_mats, seed, rows_lo=200, rows_hi=6000, dtype=np.float32):
     return As, Bs
 
 
-def test_concat_split_round_trip():
-    As, Bs = make_batch(5, 1)
-    A_cat, B_cat, lay = batch.concat(As, Bs)
-    assert A_cat.shape[0] == sum(A.shape[0] for A in As)
-    # block-diagonal product == per-member products (oracle on CPU)
-    import oracle
-    Cp, Ci, Cv = oracle.spgemm(A_cat, B_cat)
-    parts = batch.split(Cp, Ci, Cv, lay)
-    for A, B, Cm in zip(As, Bs, parts):
-        p, i, v = oracle.spgemm(A, B)
-        assert np.array_equal(Cm.ptr, p) and np.array_equal(Cm.idx, i) and np.allclose(Cm.val, v)
-
-
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_batch_graph_matches_oracle(gpu_ok, dtype):
     import torch

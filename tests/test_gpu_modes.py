@@ -1,6 +1,7 @@
 """The reference's three modes (bin/compute, bin/d2h, bin/SSpMM), budgets, chunking, the CLI
 drop-ins and the device two-phase API, all against the oracle."""
 import os
+import re
 import shutil
 import subprocess
 
@@ -108,6 +109,13 @@ def test_cli_drop_in(gpu_ok, tmp_path, exe, mode):
     assert (tmp_path / "lap.mtx.csr").exists()  # the reference's cache is written
     if exe == "debug":
         assert "- DEBUG -" in r.stdout and "Reduced to" in r.stdout
+        # the reference's DEBUG lines (Z/distributeWorkload.cpp:42-43 before the run, Z/main.cpp:113 after)
+        assert r.stdout.index("Top 10%") < r.stdout.index("Preprocessing")
+        nb = int(re.search(r"Divide Matrix A into (\d+) Blocks", r.stdout).group(1))
+        m = re.search(r"Merged (-?\d+) Blocks while Writing \(([0-9.]+)\)", r.stdout)
+        assert m, r.stdout
+        merged, ratio = int(m.group(1)), float(m.group(2))
+        assert 0 <= merged < nb and abs(ratio - merged / nb) < 1e-6
     if mode == 2:
         A = dbd_amd.read_matrix(str(src), dtype=np.float32)
         ref = reference(A)

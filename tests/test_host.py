@@ -108,6 +108,48 @@ def test_csr_cache_round_trip(tmp_path):
     assert M.val.tolist()[:2] == [1.0, 6.0]
 
 
+def test_fp64_cache_keeps_fp64_values(tmp_path):
+    """An FP64 read with the cache on must give the same values the second time (from the
+    cache) as the first (from the text): the FP64 cache is <file>.csr64 with double values, and a
+    float .csr cache written by an FP32 run is not used for FP64."""
+    src = tmp_path / "m.mtx"
+    src.write_text("%%MatrixMarket matrix coordinate real general\n3 3 4\n"
+                   "1 1 0.1\n1 3 1.000000001\n2 2 3.141592653589793\n3 1 -2.718281828459045\n")
+    M32 = dbd_amd.read_matrix(str(src), dtype=np.float32, use_cache=True)  # writes the float .csr
+    assert (tmp_path / "m.mtx.csr").exists()
+    M1 = dbd_amd.read_matrix(str(src), dtype=np.float64, use_cache=True)
+    assert (tmp_path / "m.mtx.csr64").exists()
+    M2 = dbd_amd.read_matrix(str(src), dtype=np.float64, use_cache=True)  # from .csr64 now
+    assert M1.val.tolist() == [0.1, 1.000000001, 3.141592653589793, -2.718281828459045]
+    assert np.array_equal(M1.ptr, M2.ptr) and np.array_equal(M1.idx, M2.idx)
+    assert M1.val.tolist() == M2.val.tolist()
+    assert M32.val.tolist() == np.float32([0.1, 1.000000001, 3.141592653589793, -2.718281828459045]).tolist()
+    raw = np.fromfile(tmp_path / "m.mtx.csr64", dtype=np.int32)
+    assert raw[:3].tolist() == [4, 4, 4]
+
+
+def test_read_dat_without_banner(tmp_path):
+    """make_CSR (Z/readInput.cpp:52-56) reads the first line of a .dat file as the size line:
+    there is no MatrixMarket banner to skip. The same entries as a .mtx give the same matrix."""
+    body = "3 4 5\n1 1 2.0\n1 4 1.5\n2 2 -1.0\n3 1 4.0\n3 3 0.5\n"
+    dat = tmp_path / "m.dat"
+    dat.write_text(body)
+    mtx = tmp_path / "m.mtx"
+    mtx.write_text("%%MatrixMarket matrix coordinate real general\n" + body)
+    D = dbd_amd.read_matrix(str(dat), dtype=np.float64)
+    M = dbd_amd.read_matrix(str(mtx), dtype=np.float64)
+    assert D.shape == (4, 5) == M.shape
+    assert np.array_equal(D.ptr, M.ptr) and np.array_equal(D.idx, M.idx) and np.array_equal(D.val, M.val)
+    dense = np.zeros((4, 5))
+    dense[1, 1], dense[1, 4], dense[2, 2], dense[3, 1], dense[3, 3] = 2.0, 1.5, -1.0, 4.0, 0.5
+    np.testing.assert_array_equal(D.to_dense(), dense)
+    # a .mtx without its banner loses the size line to the skip: the reader refuses it
+    bad = tmp_path / "nobanner.mtx"
+    bad.write_text(body)
+    with pytest.raises(dbd_amd.DBDError):
+        dbd_amd.read_matrix(str(bad), dtype=np.float64)
+
+
 def test_read_errors(tmp_path):
     with pytest.raises(dbd_amd.DBDError) as e:
         dbd_amd.read_matrix(str(tmp_path / "missing.mtx"))
@@ -152,3 +194,23 @@ def test_plan_blocks_vs_oracle(nr, br, seed):
         cats.count(0), cats.count(1), cats.count(2))
     for avg in (0, 64, 65, c, c + 1):
         assert dbd_amd.analyze_block(c, avg) == oracle.analyze_block(c, avg)
+
+
+def test_batch_concat_split_round_trip():
+    """The batched form's block-diagonal layout (host logic only): the product of the
+    concatenations, split per member, equals every member's own product (oracle on CPU)."""
+    from dbd_amd import batch
+    rng = np.random.default_rng(1)
+    As, Bs = [], []
+    for i in range(5):
+        n = int(np.exp(rng.uniform(np.log(200), np.log(6000))))
+        m = float(rng.uniform(4, 64))
+        As.append(dbd_amd.gen_skewed(n, m, 1.8, seed=1000 + 2 * i, dtype=np.float32))
+        Bs.append(dbd_amd.gen_skewed(n, m, 1.8, seed=1000 + 2 * i + 1, dtype=np.float32))
+    A_cat, B_cat, lay = batch.concat(As, Bs)
+    assert A_cat.shape[0] == sum(A.shape[0] for A in As)
+    Cp, Ci, Cv = oracle.spgemm(A_cat, B_cat)
+    parts = batch.split(Cp, Ci, Cv, lay)
+    for A, B, Cm in zip(As, Bs, parts):
+        p, i, v = oracle.spgemm(A, B)
+        assert np.array_equal(Cm.ptr, p) and np.array_equal(Cm.idx, i) and np.allclose(Cm.val, v)
