@@ -7,10 +7,15 @@ P = 512,000,000 products; metric GFLOPS = 2 * P / t.
 
 Windows (the reference's three binaries): kernel (`value`), kernel + D2H into pinned host
 memory, kernel + D2H + file writes. The dominant kernel's roofline is measured with HIP
-events on the stream it runs on; the CPU baseline is the oracle (C, OpenMP) on a bounded
-sample of rows, on this host's cores.
+events on the stream it runs on; the CPU baseline is the oracle (C, OpenMP) over the whole
+product, 1 warm-up + 5 runs (median), with a 1-thread figure beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes compute,d2h,io]
+`extras` (skipped with --no-extras) carries the other configurations, measured in the same run:
+R-MAT scale 22 C = A*A kernel window with the numeric pass's roofline, the R-MAT scale 18
+forward+backward / forward ratio, and the 512-member batch against per-matrix plain runs with
+the same tuned category table.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes compute,d2h,io] [--no-extras]
 """
 import argparse
 import json
@@ -47,27 +52,59 @@ def symbolic_bytes(m, nnzA, P):
     return 8 * (m + 1) + 4 * nnzA + 8 * nnzA + 4 * P + 8 * (m + 1)
 
 
-def cpu_baseline(A, B, target_s=12.0):
-    import numpy as np
+def cpu_info():
+    """Model name, sockets and logical CPUs of this host (lscpu)."""
+    import subprocess
+    out = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        return out
+    for line in txt.splitlines():
+        k, _, v = line.partition(":")
+        k, v = k.strip(), v.strip()
+        if k == "Model name":
+            out["cpu_model"] = v
+        elif k == "Socket(s)":
+            out["sockets"] = int(v) if v.isdigit() else v
+        elif k == "CPU(s)":
+            out["host_cpus"] = int(v) if v.isdigit() else v
+    return out
 
+
+def cpu_baseline(A, B, runs=5):
+    """The oracle (row-wise Gustavson, C + OpenMP, FP64) on the whole product: 1 warm-up and
+    `runs` timed runs on all usable host threads (median reported, min beside it), then one run
+    on 1 thread over the first 1/16 of A's rows (the sequential convention of the reference's MKL
+    baseline, mkl/Makefile:2)."""
+    import statistics
+
+    import dbd_amd
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = max(1, min(threads, os.cpu_count() or 1))
-    P = oracle.row_products(A.ptr, A.idx, B.ptr)
-    # calibrate on 20k rows, then size the sample for ~target_s of work
-    r0 = 20000
+    P = int(oracle.row_products(A.ptr, A.idx, B.ptr).sum())
+    oracle.spgemm(A, B, threads=threads)
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.spgemm(A, B, threads=threads)
+        ts.append(time.perf_counter() - t0)
+    med, mn = statistics.median(ts), min(ts)
+    r1 = A.shape[0] // 16
+    a1 = int(A.ptr[r1])
+    As = dbd_amd.CSR(A.ptr[:r1 + 1].copy(), A.idx[:a1], A.val[:a1], (r1, A.shape[1]))
+    P1 = int(oracle.row_products(As.ptr, As.idx, B.ptr).sum())
     t0 = time.perf_counter()
-    oracle.spgemm_rows(np.arange(r0, dtype=np.int32), A, B, threads=threads)
-    dt = time.perf_counter() - t0
-    rate = P[:r0].sum() / max(dt, 1e-9)
-    rows = int(min(A.shape[0], max(r0, target_s * rate / max(P.mean(), 1))))
-    t0 = time.perf_counter()
-    oracle.spgemm_rows(np.arange(rows, dtype=np.int32), A, B, threads=threads)
-    dt = time.perf_counter() - t0
-    prod = int(P[:rows].sum())
-    return {"value": round(2.0 * prod / dt / 1e9, 4), "unit": "GFLOPS", "cores": threads, "kind": "port",
-            "sample": f"oracle C=A*B FP64 (row-wise Gustavson, OpenMP) on rows [0, {rows}) of A with all of B "
-                      f"({prod} products, {dt:.2f} s, {threads} threads)"}
+    oracle.spgemm(As, B, threads=1)
+    t1 = time.perf_counter() - t0
+    return {"value": round(2.0 * P / med / 1e9, 4), "unit": "GFLOPS", "cores": threads, "kind": "port",
+            "sample": f"oracle C=A*B FP64 (row-wise Gustavson, OpenMP) over the whole product ({P} products), "
+                      f"1 warm-up + {runs} runs on {threads} threads: median {med:.3f} s, min {mn:.3f} s",
+            "median_s": round(med, 4), "min_s": round(mn, 4), "max_gflops": round(2.0 * P / mn / 1e9, 4),
+            "one_thread": {"gflops": round(2.0 * P1 / t1 / 1e9, 4), "s": round(t1, 3),
+                           "sample": f"rows [0, {r1}) of A with all of B ({P1} products)"},
+            **cpu_info()}
 
 
 def pinned_copy_gbps(nbytes=1 << 30, reps=3):
@@ -137,6 +174,9 @@ def main():
                     help="er2m (default headline) or one of the other named configurations")
     ap.add_argument("--batch-members", type=int, default=512)
     ap.add_argument("--rmat-scale", type=int, default=22)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other configurations' lines (R-MAT 22 kernel window, forward+backward, "
+                         "512-member batch) that the default run adds under \"extras\"")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counter passes: 3 numeric passes of the headline "
                          "workload, then 3 elementwise copies of 1 GiB (the known-size reference)")
@@ -270,6 +310,16 @@ def main():
     ctx.close()
 
     extra = {}
+    if not args.no_extras:
+        import bench_workloads as bw
+        ex = {}
+        log("extras: R-MAT scale 22 kernel window (1 warm-up + 1 run)")
+        ex["rmat22"], _ = bw.rmat_kernel_window(22, steps=1, warmup=1)
+        log("extras: forward + backward, R-MAT scale 18, FP32")
+        ex["fwd_bwd"] = bw.forward_backward(steps=3, warmup=1)
+        log("extras: 512-member batch vs per-matrix plain runs (same tuned table)")
+        ex["batch"] = bw.batch_speedup(512, steps=1)
+        extra["extras"] = ex
     if "d2h" in modes:
         try:
             extra["pinned_copy_GBps"] = round(pinned_copy_gbps(), 2)

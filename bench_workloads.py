@@ -123,6 +123,48 @@ def run_web(args):
                       "web-like 281,903^2, C=A*A, FP64, 4096-row blocks", "unit": "GFLOPS", "runs": rows}))
 
 
+def numeric_bytes(m, nnzA, P, nnzC, vsz=8):
+    """Algorithmic HBM bytes of one numeric pass: A.ptr 8(m+1), A idx+val (4+v)nnzA, B.ptr pair
+    8 nnzA, B idx+val gathers (4+v)P, C.ptr 8(m+1), C idx+val writes (4+v)nnzC."""
+    return 8 * (m + 1) + (4 + vsz) * nnzA + 8 * nnzA + (4 + vsz) * P + 8 * (m + 1) + (4 + vsz) * nnzC
+
+
+def rmat_kernel_window(scale=22, steps=1, warmup=1, A=None):
+    """R-MAT C = A*A, FP64, 512-row blocks, 64 GB device budget: the kernel window over the whole
+    product (chunks streamed within the budget), with the numeric pass's roofline (algorithmic
+    bytes over the numeric pass's device time, summed over its chunks)."""
+    import torch
+
+    import dbd_amd
+    t0 = time.perf_counter()
+    if A is None:
+        A = dbd_amd.gen_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1)
+    log(f"rmat{scale}: n={A.shape[0]} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
+    ctx = dbd_amd.Context(block_rows=512, dtype=np.float64, device_gib=64, host_gib=256, retain=False)
+    ctx.set_operands(A)
+    info = ctx.plan()
+    P = info["products"]
+    t, tm = _timed(ctx, "compute", steps, warmup)
+    info = ctx.plan_info()
+    nnzC = info["nnz"]
+    nb = numeric_bytes(A.shape[0], A.nnz, P, nnzC)
+    num_s = tm["numeric_s"]
+    res = {"workload": f"R-MAT scale {scale}, edge factor 16, (.57,.19,.19,.05), C=A*A, FP64, 512-row blocks, "
+                       f"device budget 64 GB",
+           "n": A.shape[0], "nnz_A": A.nnz, "products": P, "nnz_C": nnzC,
+           "kernel_window_ms": round(t * 1e3, 2), "gflops": round(2 * P / t / 1e9, 2), "chunks": tm["chunks"],
+           "symbolic_pass_ms": round(tm["symbolic_s"] * 1e3, 2), "numeric_pass_ms": round(num_s * 1e3, 2),
+           "numeric_roofline": {"bound": "hbm", "algorithmic_bytes": nb,
+                                "achieved": round(nb / num_s / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                                "frac": round(nb / num_s / 1e9 / 8000.0, 4)},
+           "num_rows_per_category": info["num_rows"]}
+    ctx.close()
+    torch.cuda.empty_cache()
+    log(f"rmat{scale}: kernel window {t * 1e3:.1f} ms ({2 * P / t / 1e9:.1f} GFLOPS), numeric pass "
+        f"{num_s * 1e3:.1f} ms = {nb / num_s / 1e9:.0f} GB/s")
+    return res, A
+
+
 def run_rmat22(args):
     import torch
 
@@ -130,23 +172,8 @@ def run_rmat22(args):
     import oracle
     torch.cuda.set_device(0)
     sc = args.rmat_scale
-    t0 = time.perf_counter()
-    A = dbd_amd.gen_rmat(sc, 16, 0.57, 0.19, 0.19, seed=1)
-    log(f"rmat{sc}: n={A.shape[0]} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
-    ctx = dbd_amd.Context(block_rows=512, dtype=np.float64, device_gib=64, host_gib=256, retain=False)
-    ctx.set_operands(A)
-    info = ctx.plan()
-    P = info["products"]
-    log(f"products {P}, categories {info['sym_rows']}")
-    t, tm = _timed(ctx, "compute", max(1, args.steps), min(1, args.warmup))
-    info = ctx.plan_info()
-    res = {"metric": "SpGEMM GFLOPS (2*products/s), kernel window", "workload":
-           f"R-MAT scale {sc}, edge factor 16, (.57,.19,.19,.05), C=A*A, FP64, 512-row blocks, device budget 64 GB",
-           "n": A.shape[0], "nnz_A": A.nnz, "products": P, "nnz_C": info["nnz"],
-           "kernel": {"ms": round(t * 1e3, 2), "gflops": round(2 * P / t / 1e9, 2), "chunks": tm["chunks"],
-                      "num_rows_per_category": info["num_rows"]}}
-    log(json.dumps(res["kernel"]))
-    ctx.close()
+    res, A = rmat_kernel_window(sc, max(1, args.steps), min(1, args.warmup))
+    P = res["products"]
     modes = [m for m in args.modes.split(",") if m]
     if "d2hfull" in modes:
         # the whole product with C copied to pinned host memory (chunks through a 64 GB ring):
@@ -190,7 +217,7 @@ def run_rmat22(args):
         res["cpu_sample"]["whole_product_s_est"] = round(cpu_s, 1)
         if "d2h_full" in res:
             res["end_to_end_d2h_vs_cpu"] = round(cpu_s / (res["d2h_full"]["ms"] * 1e-3), 2)
-        res["kernel_vs_cpu"] = round(cpu_s / (res["kernel"]["ms"] * 1e-3), 2)
+        res["kernel_vs_cpu"] = round(cpu_s / (res["kernel_window_ms"] * 1e-3), 2)
     print(json.dumps(res))
 
 
@@ -206,36 +233,35 @@ def _make_batch(n_mats, seed=2024, lo=1000, hi=200000):
     return As, Bs
 
 
-def run_batch(args):
+def batch_speedup(members=512, steps=1):
+    """The batched form on `members` uneven FP32 products: one captured graph (category table
+    autotuned on the concatenation) against the plain pipeline once per member with the same
+    table. C is allocated at the exact nnz the batch's symbolic pass reports."""
     import torch
 
     import oracle
     from dbd_amd import batch
-    torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    As, Bs = _make_batch(args.batch_members)
+    As, Bs = _make_batch(members)
     A_cat, B_cat, lay = batch.concat(As, Bs)
     P = int(oracle.row_products(A_cat.ptr, A_cat.idx, B_cat.ptr).sum())
     log(f"batch of {len(As)}: rows {A_cat.shape[0]}, nnz A {A_cat.nnz}, products {P}, "
         f"built in {time.perf_counter() - t0:.1f}s")
-    free, total = torch.cuda.mem_get_info()
-    need = P * 8 + (A_cat.nnz + B_cat.nnz) * 16 + A_cat.shape[0] * 80
-    if need > 0.9 * free:
-        raise SystemExit(f"batch needs ~{need / 2**30:.0f} GiB of device memory, {free / 2**30:.0f} GiB free")
     runner = batch.BatchRunner(A_cat, B_cat, block_rows=4096, tune=True)
+    del A_cat, B_cat
     runner.launch()
     torch.cuda.synchronize()
-    steps = max(1, args.steps)
     t0 = time.perf_counter()
     for _ in range(steps):
         runner.launch()
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
-    nnzC = int(runner.Cptr[-1].item())
+    nnzC, table = runner.nnz, runner.table
     runner.close()
     del runner
     torch.cuda.empty_cache()
-    ctx, run = batch.plain_each(As, Bs, block_rows=4096)
+    log(f"batch: graph {t_batch * 1e3:.1f} ms, table {table}")
+    ctx, run = batch.plain_each(As, Bs, block_rows=4096, table=table)
     run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -243,21 +269,31 @@ def run_batch(args):
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     ctx.close()
-    res = {"metric": "batched SpGEMM GFLOPS (2*products/s)", "workload":
-           f"{len(As)} independent C_i = A_i*B_i, FP32, rows log-U[1e3,2e5], mean nnz/row U[4,64], Zipf(1.8)",
-           "members": len(As), "rows": A_cat.shape[0], "products": P, "nnz_C": nnzC,
-           "batched_graph": {"ms": round(t_batch * 1e3, 2), "gflops": round(2 * P / t_batch / 1e9, 2)},
-           "plain_per_matrix": {"ms": round(t_plain * 1e3, 2), "gflops": round(2 * P / t_plain / 1e9, 2)},
-           "speedup": round(t_plain / t_batch, 3)}
+    torch.cuda.empty_cache()
+    log(f"batch: plain per-matrix {t_plain * 1e3:.1f} ms")
+    return {"workload": f"{len(As)} independent C_i = A_i*B_i, FP32, rows log-U[1e3,2e5], mean nnz/row U[4,64], "
+                        f"Zipf(1.8)",
+            "members": len(As), "products": P, "nnz_C": nnzC, "table": table,
+            "batched_graph": {"ms": round(t_batch * 1e3, 2), "gflops": round(2 * P / t_batch / 1e9, 2)},
+            "plain_per_matrix_same_table": {"ms": round(t_plain * 1e3, 2), "gflops": round(2 * P / t_plain / 1e9, 2)},
+            "speedup": round(t_plain / t_batch, 3)}
+
+
+def run_batch(args):
+    import torch
+    torch.cuda.set_device(0)
+    res = batch_speedup(args.batch_members, max(1, args.steps))
+    res["metric"] = "batched SpGEMM GFLOPS (2*products/s)"
     print(json.dumps(res))
 
 
-def run_fwdbwd(args):
+def forward_backward(steps=3, warmup=1):
+    """R-MAT scale 18 A (row-normalised adjacency + self loops) times an independent R-MAT B,
+    FP32, through torch.autograd: forward alone vs forward + backward, loss sum(C * W)."""
     import torch
 
     import dbd_amd
     from dbd_amd.autograd import spgemm_values
-    torch.cuda.set_device(0)
     A = dbd_amd.gen_rmat(18, 16, seed=1, add_identity=True, row_normalize=True, dtype=np.float32)
     B = dbd_amd.gen_rmat(18, 16, seed=2, dtype=np.float32)
     Ad, Bd = dbd_amd.DeviceCSR.from_host(A), dbd_amd.DeviceCSR.from_host(B)
@@ -269,9 +305,10 @@ def run_fwdbwd(args):
 
     Cv, Cp, Ci = fwd()
     W = torch.randn(Cv.shape, dtype=torch.float32, device="cuda")
+    nnzC = int(Ci.numel())
+    del Cv, Cp, Ci
     P = int(np.diff(B.ptr)[A.idx].sum())
-    steps = max(1, args.steps)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         with torch.no_grad():
             fwd()
     torch.cuda.synchronize()
@@ -281,7 +318,7 @@ def run_fwdbwd(args):
             fwd()
     torch.cuda.synchronize()
     t_f = (time.perf_counter() - t0) / steps
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         av.grad = bv.grad = None
         c, _, _ = fwd()
         (c * W).sum().backward()
@@ -293,9 +330,18 @@ def run_fwdbwd(args):
         (c * W).sum().backward()
     torch.cuda.synchronize()
     t_fb = (time.perf_counter() - t0) / steps
-    res = {"metric": "forward+backward / forward time", "workload":
-           "R-MAT scale 18 (row-normalised + self loops) x independent R-MAT scale 18, FP32, autograd",
-           "nnz_A": A.nnz, "nnz_B": B.nnz, "products": P, "nnz_C": int(Ci.numel()),
-           "forward_ms": round(t_f * 1e3, 2), "fwd_bwd_ms": round(t_fb * 1e3, 2),
-           "ratio": round(t_fb / t_f, 3), "forward_gflops": round(2 * P / t_f / 1e9, 2)}
+    del W, c, av, bv, Ad, Bd
+    torch.cuda.empty_cache()
+    log(f"fwd {t_f * 1e3:.1f} ms, fwd+bwd {t_fb * 1e3:.1f} ms")
+    return {"workload": "R-MAT scale 18 (row-normalised + self loops) x independent R-MAT scale 18, FP32, autograd",
+            "nnz_A": A.nnz, "nnz_B": B.nnz, "products": P, "nnz_C": nnzC,
+            "forward_ms": round(t_f * 1e3, 2), "fwd_bwd_ms": round(t_fb * 1e3, 2),
+            "ratio": round(t_fb / t_f, 3), "forward_gflops": round(2 * P / t_f / 1e9, 2)}
+
+
+def run_fwdbwd(args):
+    import torch
+    torch.cuda.set_device(0)
+    res = forward_backward(max(1, args.steps), args.warmup)
+    res["metric"] = "forward+backward / forward time"
     print(json.dumps(res))

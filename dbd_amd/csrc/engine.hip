@@ -28,22 +28,9 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "pipeline.h"
 
 using namespace dbd;
-
-#define HIPCHK(expr)                                                                                   \
-  do {                                                                                                 \
-    hipError_t e_ = (expr);                                                                            \
-    if (e_ != hipSuccess)                                                                              \
-      return set_error(DBD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                       __LINE__);                                                                      \
-  } while (0)
-
-#define CHK(expr)                        \
-  do {                                   \
-    dbd_status s_ = (expr);              \
-    if (s_ != DBD_OK) return s_;         \
-  } while (0)
 
 namespace {
 
@@ -103,17 +90,13 @@ struct HostBuf {
   T* as() const { return (T*)p; }
 };
 
-// ---- category tables -------------------------------------------------------------------
-struct Tables {
-  CatTable sym_ct, num_ct;
-  KernelShape sym_k[kMaxCats], num_k[kMaxCats];
-  int id = 0;
-};
+}  // namespace
 
+// ---- category tables -------------------------------------------------------------------
 // Default table. Symbolic key: min(P_i, ncols) (a bound on nnz_i); numeric key: nnz_i.
 // Hash tables are sized so the load stays <= `load`; the one-wave symbolic tables (keys only,
 // 4 bytes a slot, up to 1024 slots) to half of that, which shortens their probe sequences.
-Tables default_tables(int g_small, int g_big, double load) {
+Tables dbd::default_tables(int g_small, int g_big, double load) {
   Tables t;
   const int Ts[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
   const int NTs[] = {64, 64, 64, 64, 128, 256, 512, 1024, 1024};
@@ -139,6 +122,8 @@ Tables default_tables(int g_small, int g_big, double load) {
   t.num_ct.n = n;
   return t;
 }
+
+namespace {
 
 std::vector<Tables>& candidate_tables() {
   static std::vector<Tables> c = [] {
@@ -997,42 +982,61 @@ dbd_status dbd_backward(dbd_ctx* c, const int64_t* d_Cptr, const int32_t* d_Cidx
 
 }  // extern "C"
 
-// ---- batched form ----------------------------------------------------------------------
-static int persistent_grid(dbd_ctx* c, const KernelShape& k) {
-  if (k.T == 0) return c->num_cu * 2;
-  return c->num_cu * (k.NT <= 64 ? 8 : 2);
+// ---- whole pipeline without host synchronisation (batched graph, no-allocation entry) -----
+static int persistent_grid(int num_cu, const KernelShape& k) {
+  if (k.T == 0) return num_cu * 2;
+  return num_cu * (k.NT <= 64 ? 8 : 2);
 }
 
 // The whole pipeline (row analysis, bins, symbolic, scan, bins, numeric) on stream s with
-// persistent category kernels; no host synchronisation, so it can be captured.
-static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval) {
-  const int64_t nr = c->nr;
-  launch_fill_i32(c->err.as<int32_t>(), 1, 0, s);
-  launch_row_products(c->A, c->B, c->P.as<int64_t>(), c->nnz.as<int64_t>(), (int64_t*)c->A.bstart,
-                      (int32_t*)c->A.blen, s);
-  const CatTable& ct = c->tb.sym_ct;
-  launch_bin_count(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->cnt.as<int64_t>(), s);
-  launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_sym.as<int64_t>(), (int64_t)ct.n * c->nwg, c->tmp.as<int64_t>(), s);
-  launch_bin_scatter(c->P.as<int64_t>(), nr, c->B.ncols, ct, c->offs_sym.as<int64_t>(), c->rows_sym.as<int32_t>(), s);
-  const int64_t* so = c->offs_sym.as<int64_t>();
+// persistent category kernels that read their row ranges from device memory: no host
+// synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
+// the numeric kernels drain empty ranges and the capacity bit is set in the error word.
+void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu,
+                             int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s) {
+  const int64_t nr = w.nr;
+  A.bstart = w.bstart;
+  A.blen = w.blen;
+  launch_fill_i32(w.err, 1, 0, s);
+  launch_row_products(A, B, w.P, w.nnz, w.bstart, w.blen, s);
+  const CatTable& ct = tb.sym_ct;
+  launch_bin_count(w.P, nr, B.ncols, ct, w.cnt, s);
+  launch_exclusive_scan(w.cnt, w.offs_sym, (int64_t)ct.n * w.nwg, w.tmp, s);
+  launch_bin_scatter(w.P, nr, B.ncols, ct, w.offs_sym, w.rows_sym, s);
   for (int k = 1; k < ct.n; ++k)
-    launch_symbolic(c->tb.sym_k[k], c->rows_sym.as<int32_t>(), so + (int64_t)k * c->nwg, so + (int64_t)(k + 1) * c->nwg,
-                    0, persistent_grid(c, c->tb.sym_k[k]), c->A, c->B, c->nnz.as<int64_t>(), c->cursor.as<int64_t>(),
-                    c->err.as<int>(), s);
-  launch_exclusive_scan(c->nnz.as<int64_t>(), d_Cptr, nr, c->tmp.as<int64_t>(), s);
-  const CatTable& nt = c->tb.num_ct;
-  launch_bin_count(c->nnz.as<int64_t>(), nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s);
-  launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_num.as<int64_t>(), (int64_t)nt.n * c->nwg, c->tmp.as<int64_t>(), s);
-  launch_bin_scatter(c->nnz.as<int64_t>(), nr, INT64_MAX, nt, c->offs_num.as<int64_t>(), c->rows_num.as<int32_t>(), s);
-  // C was sized by the symbolic pass of dbd_batch_prepare; should the operands' structure have
-  // changed since, a larger output is refused on the device instead of overrunning C
-  launch_capacity_guard(d_Cptr + nr, c->batch_cap, c->offs_num.as<int64_t>(), (int64_t)nt.n * c->nwg + 1,
-                        c->err.as<int>(), s);
-  const int64_t* no = c->offs_num.as<int64_t>();
+    launch_symbolic(tb.sym_k[k], w.rows_sym, w.offs_sym + (int64_t)k * w.nwg, w.offs_sym + (int64_t)(k + 1) * w.nwg, 0,
+                    persistent_grid(num_cu, tb.sym_k[k]), A, B, w.nnz, w.cursor, w.err, s);
+  launch_exclusive_scan(w.nnz, d_Cptr, nr, w.tmp, s);
+  const CatTable& nt = tb.num_ct;
+  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s);
+  launch_exclusive_scan(w.cnt, w.offs_num, (int64_t)nt.n * w.nwg, w.tmp, s);
+  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s);
+  launch_capacity_guard(d_Cptr + nr, cap, w.offs_num, (int64_t)nt.n * w.nwg + 1, w.err, s);
   for (int k = 1; k < nt.n; ++k)
-    launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), no + (int64_t)k * c->nwg,
-                   no + (int64_t)(k + 1) * c->nwg, 0, persistent_grid(c, c->tb.num_k[k]), c->A, c->B, d_Cptr, 0,
-                   d_Cidx, d_Cval, c->cursor.as<int64_t>(), c->err.as<int>(), s);
+    launch_numeric(dtype, tb.num_k[k], w.rows_num, w.offs_num + (int64_t)k * w.nwg, w.offs_num + (int64_t)(k + 1) * w.nwg,
+                   0, persistent_grid(num_cu, tb.num_k[k]), A, B, d_Cptr, 0, d_Cidx, d_Cval, w.cursor, w.err, s);
+}
+
+// The batch's pipeline on the context's workspace. C was sized by the symbolic pass of
+// dbd_batch_prepare; should the operands' structure have changed since, a larger output is
+// refused on the device instead of overrunning C.
+static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval) {
+  Work w;
+  w.P = c->P.as<int64_t>();
+  w.nnz = c->nnz.as<int64_t>();
+  w.cnt = c->cnt.as<int64_t>();
+  w.offs_sym = c->offs_sym.as<int64_t>();
+  w.offs_num = c->offs_num.as<int64_t>();
+  w.tmp = c->tmp.as<int64_t>();
+  w.bstart = c->bdesc_s.as<int64_t>();
+  w.cursor = c->cursor.as<int64_t>();
+  w.rows_sym = c->rows_sym.as<int32_t>();
+  w.rows_num = c->rows_num.as<int32_t>();
+  w.blen = c->bdesc_l.as<int32_t>();
+  w.err = c->err.as<int>();
+  w.nr = c->nr;
+  w.nwg = c->nwg;
+  enqueue_pipeline(w, c->tb, c->A, c->B, c->o.dtype, c->num_cu, d_Cptr, d_Cidx, d_Cval, c->batch_cap, s);
   return DBD_OK;
 }
 

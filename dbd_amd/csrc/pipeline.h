@@ -1,0 +1,53 @@
+// Internal declarations shared by the engine's translation units (not part of the C ABI):
+// HIP status macros, category tables, and the capturable whole pipeline over raw workspace.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return set_error(DBD_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                                      \
+  } while (0)
+
+#define CHK(expr)                        \
+  do {                                   \
+    dbd_status s_ = (expr);              \
+    if (s_ != DBD_OK) return s_;         \
+  } while (0)
+
+namespace dbd {
+
+// A category table: thresholds and kernel shapes of the symbolic and numeric categories.
+struct Tables {
+  CatTable sym_ct, num_ct;
+  KernelShape sym_k[kMaxCats], num_k[kMaxCats];
+  int id = 0;
+};
+
+// Symbolic key min(P_i, ncols), numeric key nnz_i; hash tables sized so the load stays <= `load`
+// (one-wave symbolic tables at half of that); G lanes per A entry for small / large tables.
+Tables default_tables(int g_small, int g_big, double load);
+
+// Raw device workspace of one pipeline run: per-row arrays (nr + 1), bin counters and offsets
+// (kMaxCats per 256-row tile), scan partials, per-A-nonzero B-row descriptors and long-row
+// cursors, and the error word (followed by the sink degenerate rows are written to).
+struct Work {
+  int64_t *P, *nnz, *cnt, *offs_sym, *offs_num, *tmp, *bstart, *cursor;
+  int32_t *rows_sym, *rows_num, *blen;
+  int* err;
+  int64_t nr, nwg;
+};
+
+// The whole pipeline (row analysis, bins, symbolic, scan, bins, numeric) on stream s with
+// persistent category kernels that read their row ranges from device memory: no host
+// synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
+// the numeric kernels drain empty ranges and the capacity bit is set in the error word.
+void enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu, int64_t* d_Cptr,
+                      int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s);
+
+}  // namespace dbd
