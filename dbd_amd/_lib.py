@@ -92,6 +92,8 @@ SIGNATURES = {
     "dbd_autotune": (I32, [P, C.POINTER(I32)]),
     "dbd_table_count": (I32, []),
     "dbd_set_table": (I32, [P, I32]),
+    "dbd_noalloc_scratch_bytes": (C.c_size_t, [I64, I64]),
+    "dbd_spgemm_noalloc": (I32, [CSRP, CSRP, P, P, P, P, I64, C.POINTER(I64)]),
     "dbd_plan_blocks": (I32, [P, I64, I32, P, P, C.POINTER(dbd_plan_info)]),
     "dbd_analyze_block": (I32, [I64, I64]),
     "dbd_read_matrix": (I32, [C.c_char_p, I32, I32, CSRP]),

@@ -53,10 +53,11 @@ def build(verbose: bool = True, lib_out: str | None = None, defines: list[str] |
     objs = {
         "kernels.o": [HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, "kernels.hip")],
         "engine.o": [HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, "engine.hip")],
+        "noalloc.o": [HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, "noalloc.hip")],
         "host.o": [CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "host.cpp")],
     }
     jobs = []
-    with ThreadPoolExecutor(max_workers=3) as ex:
+    with ThreadPoolExecutor(max_workers=4) as ex:
         for name, cmd in objs.items():
             jobs.append(ex.submit(_run, cmd + ["-o", os.path.join(bdir, name)]))
         for j in jobs:

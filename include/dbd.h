@@ -182,6 +182,23 @@ dbd_status dbd_autotune(dbd_ctx* ctx, int32_t* chosen);
 int32_t dbd_table_count(void);
 dbd_status dbd_set_table(dbd_ctx* ctx, int32_t table_id);
 
+/* ---- entry point that allocates nothing ------------------------------------------- */
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): C = A*B for device views
+ * A and B (B NULL -> A*A) on the caller's stream (NULL: the default stream), with the whole
+ * pipeline's workspace in one fixed array of DBD_NOALLOC_SCRATCH_BYTES of device memory that the
+ * library holds on each GPU it runs on until the process ends; no hipMalloc, hipFree or other
+ * allocating call is made. The caller passes Cptr (nrows+1 int64) and Cidx / Cval with
+ * `capacity` entries; *nnz_out receives nnz(C). If nnz(C) > capacity nothing is written to
+ * Cidx / Cval and DBD_ERR_BUDGET is returned (*nnz_out says how much is needed). A product whose
+ * workspace exceeds the array (dbd_noalloc_scratch_bytes(nrows, nnz(A)); about 26 bytes per row
+ * and 28 per nonzero of A) is refused with DBD_ERR_BUDGET before any device work. Calls share
+ * their GPU's array: calls that overlap on one GPU race. Synchronises the stream before returning. */
+#define DBD_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+size_t dbd_noalloc_scratch_bytes(int64_t nrows, int64_t nnz_a);
+dbd_status dbd_spgemm_noalloc(const dbd_csr* A, const dbd_csr* B, void* hip_stream, int64_t* d_Cptr,
+                              int32_t* d_Cidx, void* d_Cval, int64_t capacity, int64_t* nnz_out);
+
 /* ---- host-only logic (no device needed) ------------------------------------------ */
 /* Blocks from per-row products (divide_Blocks, Z/main.cpp:259-296) and reference criteria
  * (Z/distributeWorkload.cpp:249-270). block_ptr has num_blocks+1 entries, block_products
