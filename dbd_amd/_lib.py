@@ -76,6 +76,7 @@ SIGNATURES = {
     "dbd_create": (I32, [C.POINTER(dbd_opts), C.POINTER(P)]),
     "dbd_destroy": (I32, [P]),
     "dbd_set_stream": (I32, [P, P]),
+    "dbd_use_own_stream": (I32, [P]),
     "dbd_set_operands": (I32, [P, CSRP, CSRP]),
     "dbd_plan": (I32, [P, C.POINTER(dbd_plan_info)]),
     "dbd_run": (I32, [P, I32, C.POINTER(dbd_timing)]),

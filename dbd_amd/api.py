@@ -244,9 +244,13 @@ class Context:
         return self._h
 
     def set_stream(self, stream) -> None:
-        """stream: a torch.cuda.Stream, a raw hipStream_t int, or None (own stream)."""
-        raw = None if stream is None else int(getattr(stream, "cuda_stream", stream))
-        check(lib.dbd_set_stream(self._h, raw))
+        """stream: a torch.cuda.Stream or a raw hipStream_t int (0 is the default stream, used as
+        such, so the work stays ordered with torch's default-stream work), or None (a stream the
+        context owns)."""
+        if stream is None:
+            check(lib.dbd_use_own_stream(self._h))
+            return
+        check(lib.dbd_set_stream(self._h, int(getattr(stream, "cuda_stream", stream))))
 
     def set_operands(self, A, B=None) -> None:
         va = A.view()

@@ -276,6 +276,10 @@ struct dbd_ctx {
   bool planned = false, symb = false;
   int64_t nr = 0, nb = 0, nwg = 0;
   DevBuf P, Pptr, nnz, Cptr, rows_sym, rows_num, cnt, offs_sym, offs_num, tmp, cursor, blk, err, bdesc_s, bdesc_l;
+  // band index of B's long rows (for the long-row window kernels), built once per operand set
+  DevBuf band_hist, band_flag, band_tmp, band_slot, band_row, band;
+  bool band_ready = false;
+  int64_t band_minlen = 0, band_rows = 0, band_nb1 = 0;
   std::vector<int64_t> h_blk, h_blockP, h_blockC, h_offs_num, h_sym_tot;
   dbd_plan_info info{};
 
@@ -317,6 +321,54 @@ dbd_status check_err(dbd_ctx* c, const char* phase) {
   if (e != 0)
     return set_error(DBD_ERR_INTERNAL, "%s: device consistency check failed (flags 0x%x)", phase, e);
   return DBD_OK;
+}
+
+// Band index of B's long rows: the rows of at least `minlen` entries, minlen the smallest power
+// of two from 16 up whose rows' index fits min(1 GiB, device budget / 16). Built once per
+// operand set (B does not change between runs).
+dbd_status build_band(dbd_ctx* c) {
+  hipStream_t s = c->st;
+  const int64_t nB = c->B.nrows;
+  c->band_ready = true;
+  c->band_rows = 0;
+  if (nB <= 0) return DBD_OK;
+  CHK(c->band_hist.ensure(8 * 32));
+  launch_len_hist(c->B.ptr, nB, c->band_hist.as<int64_t>(), s);
+  int64_t hist[32];
+  HIPCHK(hipMemcpyAsync(hist, c->band_hist.p, sizeof(hist), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int64_t nb1 = band_count(c->B.ncols);
+  const int64_t cap = std::min<int64_t>((int64_t)1 << 30, (c->o.device_budget_mib << 20) / 16);
+  int j = 4;
+  while (j < 31 && hist[j] * nb1 * 4 > cap) ++j;
+  if (hist[j] == 0 || hist[j] * nb1 * 4 > cap) return DBD_OK;  // nothing to band: cursor walks only
+  const int64_t H = hist[j];
+  CHK(c->band_flag.ensure(8 * (size_t)(nB + 1)));
+  CHK(c->band_tmp.ensure(8 * (size_t)scan_tmp_elems(nB + 1)));
+  CHK(c->band_slot.ensure(4 * (size_t)nB));
+  CHK(c->band_row.ensure(4 * (size_t)H));
+  CHK(c->band.ensure(4 * (size_t)(H * nb1)));
+  launch_heavy_slots(c->B.ptr, nB, (int64_t)1 << j, c->band_flag.as<int64_t>(), c->band_tmp.as<int64_t>(),
+                     c->band_slot.as<int32_t>(), c->band_row.as<int32_t>(), s);
+  launch_band_index(c->band_row.as<int32_t>(), H, c->B.ptr, c->B.idx, nb1, c->band.as<int32_t>(), s);
+  HIPCHK(hipGetLastError());
+  c->band_minlen = (int64_t)1 << j;
+  c->band_rows = H;
+  c->band_nb1 = nb1;
+  return DBD_OK;
+}
+
+// B as the kernels see it: with its band index when one was built
+void attach_band(dbd_ctx* c) {
+  if (c->band_rows > 0) {
+    c->B.hslot = c->band_slot.as<int32_t>();
+    c->B.band = c->band.as<int32_t>();
+    c->B.nb1 = c->band_nb1;
+  } else {
+    c->B.hslot = nullptr;
+    c->B.band = nullptr;
+    c->B.nb1 = 0;
+  }
 }
 
 // Row analysis, int64 prefix, blocks, reference criteria, symbolic bins.
@@ -387,6 +439,9 @@ dbd_status plan(dbd_ctx* c) {
     in.num_rows[i] = -1;
     in.num_limit[i] = c->tb.num_ct.lim[i];
   }
+  // long rows (the last symbolic category) read B's band index
+  if (c->h_sym_tot[(size_t)ct.n] - c->h_sym_tot[(size_t)ct.n - 1] > 0 && !c->band_ready) CHK(build_band(c));
+  attach_band(c);
   c->planned = true;
   c->symb = false;
   return DBD_OK;
@@ -487,6 +542,8 @@ int64_t fixed_device_bytes(dbd_ctx* c) {
   s += (int64_t)c->P.bytes + c->Pptr.bytes + c->nnz.bytes + c->Cptr.bytes + c->rows_sym.bytes + c->rows_num.bytes;
   s += (int64_t)c->cnt.bytes + c->offs_sym.bytes + c->offs_num.bytes + c->tmp.bytes + c->cursor.bytes;
   s += (int64_t)c->bdesc_s.bytes + c->bdesc_l.bytes;
+  s += (int64_t)c->band_hist.bytes + c->band_flag.bytes + c->band_tmp.bytes + c->band_slot.bytes + c->band_row.bytes +
+       c->band.bytes;
   return s;
 }
 
@@ -782,10 +839,11 @@ dbd_status dbd_create(const dbd_opts* opts, dbd_ctx** out) {
 
 dbd_status dbd_destroy(dbd_ctx* c) {
   if (!c) return DBD_OK;
-  if (c->st) (void)hipStreamSynchronize(c->st);
+  (void)hipStreamSynchronize(c->st);  // a null st is the default stream
   if (c->cst) (void)hipStreamSynchronize(c->cst);
   for (auto* b : {&c->P, &c->Pptr, &c->nnz, &c->Cptr, &c->rows_sym, &c->rows_num, &c->cnt, &c->offs_sym, &c->offs_num,
-                  &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng, &c->bdesc_s, &c->bdesc_l})
+                  &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng, &c->bdesc_s, &c->bdesc_l, &c->band_hist,
+                  &c->band_flag, &c->band_tmp, &c->band_slot, &c->band_row, &c->band})
     b->release();
   for (int i = 0; i < 3; ++i) {
     c->upA[i].release();
@@ -813,18 +871,27 @@ dbd_status dbd_destroy(dbd_ctx* c) {
   return DBD_OK;
 }
 
+// The caller's stream, taken as it is: NULL is the HIP default stream, the one a framework's
+// default-stream work (torch's allocations, fills, elementwise ops) is ordered on.
 dbd_status dbd_set_stream(dbd_ctx* c, void* stream) {
   clear_error();
   if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
   HIPCHK(hipStreamSynchronize(c->st));
   if (c->own_st) (void)hipStreamDestroy(c->st);
-  if (stream) {
-    c->st = (hipStream_t)stream;
-    c->own_st = false;
-  } else {
-    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-    c->own_st = true;
-  }
+  c->st = (hipStream_t)stream;
+  c->own_st = false;
+  return DBD_OK;
+}
+
+dbd_status dbd_use_own_stream(dbd_ctx* c) {
+  clear_error();
+  if (!c) return set_error(DBD_ERR_INVALID, "null ctx");
+  if (c->own_st) return DBD_OK;
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipStream_t st = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  c->st = st;
+  c->own_st = true;
   return DBD_OK;
 }
 
@@ -838,6 +905,8 @@ dbd_status dbd_set_operands(dbd_ctx* c, const dbd_csr* A, const dbd_csr* B) {
   c->ops = false;
   c->planned = c->symb = false;
   c->tuned_set = c->table_explicit;
+  c->band_ready = false;
+  c->band_rows = 0;
   c->dev_result = c->host_result = false;
   const double t0 = now_s();
   CHK(upload(c, A, c->A, c->upA, c->a_bytes));
@@ -1066,6 +1135,7 @@ dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* 
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
     c->graph = nullptr;
+    if (!c->st) return set_error(DBD_ERR_STATE, "dbd_batch_launch: the default stream cannot be captured; set another");
     HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
     dbd_status st = batch_enqueue(c, c->st, d_Cptr, d_Cidx, d_Cval);

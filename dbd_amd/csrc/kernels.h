@@ -28,6 +28,11 @@ struct Mat {
   // row analysis in A's order so later passes read it sequentially instead of gathering B.ptr.
   const int64_t* bstart = nullptr;
   const int32_t* blen = nullptr;
+  // For B only: the band index of its long rows (launch_heavy_slots / launch_band_index), or null.
+  // hslot[k] is row k's slot (-1: not banded); band + slot * nb1 are the slot's band positions.
+  const int32_t* hslot = nullptr;
+  const int32_t* band = nullptr;
+  int64_t nb1 = 0;
 };
 
 // One category's kernel shape, chosen by the planner/autotuner.
@@ -51,6 +56,24 @@ void launch_gather_strided(const int64_t* ptr, int64_t n, int64_t stride, int64_
 // offs[0 .. n), which empties every bin range that the following row kernels read.
 constexpr int kErrCapacityBit = 16;
 void launch_capacity_guard(const int64_t* total, int64_t cap, int64_t* offs, int64_t n, int* err, hipStream_t s);
+
+// ---- band index of long B rows ----
+// Columns per band; band index rows have band_count(ncols) entries (a multiple of kBandPad
+// bands, so every aligned long-row window lies inside, plus one entry holding the row length).
+constexpr int kBandShift = 12;
+constexpr int kBandPad = 32;
+int64_t band_count(int64_t ncols);
+// hist[j] (32 int64) = number of B rows with at least 2^j entries.
+void launch_len_hist(const int64_t* Bptr, int64_t nrows, int64_t* hist, hipStream_t s);
+// slot[k] = index of B row k among the rows of at least minlen entries (-1 for the others),
+// slot_row[s] = the row of slot s. flag (nrows + 1 int64) and tmp (scan_tmp_elems(nrows + 1)) are
+// scratch; flag[nrows] receives the number of slots.
+void launch_heavy_slots(const int64_t* Bptr, int64_t nrows, int64_t minlen, int64_t* flag, int64_t* tmp,
+                        int32_t* slot, int32_t* slot_row, hipStream_t s);
+// band[s * nb1 + b] = position (from the row start) of slot s's first entry with column >= b * 4096;
+// band[s * nb1 + nb1 - 1] = the row's length.
+void launch_band_index(const int32_t* slot_row, int64_t nslots, const int64_t* Bptr, const int32_t* Bidx, int64_t nb1,
+                       int32_t* band, hipStream_t s);
 
 // ---- planning ----
 // P[i] = sum_{k in A_i} len(B_k); nnz[i] = 0 where P[i] == 0; per A nonzero j with column k:

@@ -796,6 +796,93 @@ void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, i
 }
 
 // ------------------------------------------------------------------------------------
+// band index of the long B rows (for the long-row window kernels)
+// ------------------------------------------------------------------------------------
+// B rows of at least `minlen` entries ("heavy" rows) get a slot; for each, the position (from the
+// row's start) of its first entry at or past every multiple of 4096 columns, over nb1 entries,
+// the last of which (past every column) holds the row's length. A long-row window of aligned
+// columns then reads its exact element range from two entries instead of walking the row.
+__global__ void k_heavy_flag(const int64_t* __restrict__ Bptr, int64_t nrows, int64_t minlen, int64_t* __restrict__ flag) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += (int64_t)gridDim.x * blockDim.x)
+    flag[k] = (Bptr[k + 1] - Bptr[k]) >= minlen ? 1 : 0;
+}
+
+__global__ void k_heavy_slot(const int64_t* __restrict__ flag_scan, const int64_t* __restrict__ Bptr, int64_t nrows,
+                             int64_t minlen, int32_t* __restrict__ slot, int32_t* __restrict__ slot_row) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += (int64_t)gridDim.x * blockDim.x) {
+    const bool heavy = (Bptr[k + 1] - Bptr[k]) >= minlen;
+    const int32_t sl = (int32_t)flag_scan[k];
+    slot[k] = heavy ? sl : -1;
+    if (heavy) slot_row[sl] = (int32_t)k;
+  }
+}
+
+// One workgroup per heavy row: entry q of band position b is written by the element that opens
+// band b (the first element whose band is >= b), bands past the last element by the tail.
+__global__ __launch_bounds__(256) void k_band_index(const int32_t* __restrict__ slot_row, int64_t nslots,
+                                                    const int64_t* __restrict__ Bptr, const int32_t* __restrict__ Bidx,
+                                                    int64_t nb1, int32_t* __restrict__ band) {
+  for (int64_t sl = blockIdx.x; sl < nslots; sl += gridDim.x) {
+    const int k = slot_row[sl];
+    const int64_t b0 = Bptr[k];
+    const int32_t L = (int32_t)(Bptr[k + 1] - b0);
+    int32_t* out = band + sl * nb1;
+    for (int32_t q = threadIdx.x; q < L; q += blockDim.x) {
+      const int bq = Bidx[b0 + q] >> kBandShift;
+      const int bp = q > 0 ? (Bidx[b0 + q - 1] >> kBandShift) : -1;
+      for (int b = bp + 1; b <= bq; ++b) out[b] = q;
+    }
+    const int blast = L > 0 ? (Bidx[b0 + L - 1] >> kBandShift) : -1;
+    for (int64_t b = blast + 1 + threadIdx.x; b < nb1; b += blockDim.x) out[b] = L;
+  }
+}
+
+// hist[j] = number of B rows with at least 2^j entries, j = 0 .. 31 (hist zeroed by the caller)
+__global__ void k_len_hist(const int64_t* __restrict__ Bptr, int64_t nrows, int64_t* __restrict__ hist) {
+  __shared__ int h[32];
+  if (threadIdx.x < 32) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nrows; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t L = Bptr[k + 1] - Bptr[k];
+    if (L > 0) atomicAdd(&h[63 - __clzll((unsigned long long)L)], 1);  // floor(log2 L)
+  }
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    int64_t c = 0;
+    for (int j = threadIdx.x; j < 32; ++j) c += h[j];  // rows with floor(log2 L) >= threadIdx.x
+    if (c) atomicAdd((unsigned long long*)&hist[threadIdx.x], (unsigned long long)c);
+  }
+}
+
+void launch_len_hist(const int64_t* Bptr, int64_t nrows, int64_t* hist, hipStream_t s) {
+  launch_fill_i64(hist, 32, 0, s);
+  if (nrows <= 0) return;
+  int64_t g = (nrows + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_len_hist, dim3((unsigned)g), dim3(256), 0, s, Bptr, nrows, hist);
+}
+
+int64_t band_count(int64_t ncols) {
+  const int64_t nb = (ncols + (1 << kBandShift) - 1) >> kBandShift;
+  return (nb + kBandPad - 1) / kBandPad * kBandPad + 1;
+}
+
+void launch_heavy_slots(const int64_t* Bptr, int64_t nrows, int64_t minlen, int64_t* flag, int64_t* tmp,
+                        int32_t* slot, int32_t* slot_row, hipStream_t s) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_heavy_flag, dim3(fill_grid(nrows)), dim3(256), 0, s, Bptr, nrows, minlen, flag);
+  launch_exclusive_scan(flag, flag, nrows, tmp, s);
+  hipLaunchKernelGGL(k_heavy_slot, dim3(fill_grid(nrows)), dim3(256), 0, s, flag, Bptr, nrows, minlen, slot, slot_row);
+}
+
+void launch_band_index(const int32_t* slot_row, int64_t nslots, const int64_t* Bptr, const int32_t* Bidx, int64_t nb1,
+                       int32_t* band, hipStream_t s) {
+  if (nslots <= 0) return;
+  const int64_t g = nslots < 65536 ? nslots : 65536;
+  hipLaunchKernelGGL(k_band_index, dim3((unsigned)g), dim3(256), 0, s, slot_row, nslots, Bptr, Bidx, nb1, band);
+}
+
+// ------------------------------------------------------------------------------------
 // binning: stable counting sort of row ids by category
 // ------------------------------------------------------------------------------------
 constexpr int kBinTile = 256;
@@ -937,6 +1024,22 @@ __global__ __launch_bounds__(wg_of<NT>()) void k_symbolic_hash(
     gsync<NT>();
   }
 }
+
+// Per-row records of the banded long-row windows, 16 bytes each (the workspace holds 16 bytes per
+// A nonzero): an entry with a banded B row (start of its B row, its slot, its index in the row),
+// and one without (position from its B row's start, index in the row, next column, entries left).
+struct HRec {
+  int64_t b0;
+  int32_t slot;
+  int32_t e;
+};
+struct LRec {
+  int32_t prel;
+  int32_t e;
+  int32_t nc;
+  int32_t rem;
+};
+static_assert(sizeof(HRec) == 16 && sizeof(LRec) == 16, "records share the 16-byte workspace slots");
 
 // Cursor of one A entry of a long row in a column sweep: position in B and entries left of
 // its B row, and the column at that position (INT_MAX when the row is exhausted).
@@ -1486,13 +1589,15 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 // (16 bytes per A nonzero). Both paths leave the value array and the bitmap clear.
 constexpr int kNumLongStates = 512;
 
+
 template <typename V, int NT, int BW, int VC>
-__global__ __launch_bounds__(NT) void k_numeric_long(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_numeric_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
     const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
     const V* __restrict__ Aval, const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
     const int64_t* __restrict__ Cptr, int64_t cbase, int32_t* __restrict__ Cidx, V* __restrict__ Cval,
-    LongState* __restrict__ state, int* __restrict__ err) {
+    LongState* __restrict__ state, int* __restrict__ err, const int32_t* __restrict__ Aidx,
+    const int32_t* __restrict__ hslot, const int32_t* __restrict__ band, int64_t nb1) {
   constexpr int NWORD = BW / 32;
   constexpr int WPT = NWORD / NT;  // bitmap words per thread
   constexpr int OWORD = VC / 32;   // bitmap words of a dense window
@@ -1511,6 +1616,9 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
   __shared__ int red[NW], red_cnt[NW];
   __shared__ long long redl[NW];
   __shared__ long long s_hi2;
+  __shared__ int s_nh, s_nl;
+  static_assert(2 * NT * (8 + (int)sizeof(V) + 4) <= NWORD * 2 + kNumLongStates * 16,
+                "banded windows: range tables exceed the LDS they share");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
   for (int q = t; q < VC; q += NT) dv[q] = V(0);
@@ -1716,6 +1824,292 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
         }
       }
     } else {
+      bool banded_done = false;  // uniform
+      if (band != nullptr) {
+      // ================= banded dense windows =================
+      // Windows are aligned blocks of VC columns. An entry whose B row is banded knows the exact
+      // range of its elements in each window from the band index; the window's elements of all
+      // banded entries are gathered element-parallel over the workgroup, one per lane. The other
+      // entries (B rows below the banding threshold) keep a cursor, one lane each, four elements
+      // per load. Per-entry state stays in registers (up to KE banded and KE other entries per
+      // thread), and each window's element loads are issued together with the next window's band
+      // end and next column, so a window costs one dependent round trip to memory. Rows with more
+      // entries of either kind take the cursor windows below.
+      constexpr int LOGVC = __builtin_ctz(VC);
+      constexpr int BPW = VC >> kBandShift;  // bands per window
+      constexpr int KE = 2;
+      HRec* hrec = reinterpret_cast<HRec*>(state + a0);
+      LRec* lrec = reinterpret_cast<LRec*>(state + a0);
+      // chunk arrays in the word-prefix and state LDS of the other paths, which both rewrite it
+      // before use (the bitmap words past the dense window's must stay clear for the two walks)
+      unsigned char* reg = lds + DVB + NWORD * 4;
+      int64_t* hb0 = reinterpret_cast<int64_t*>(reg);                        // KE * NT
+      V* hva = reinterpret_cast<V*>(reg + KE * NT * 8);                      // KE * NT
+      int* hS = reinterpret_cast<int*>(reg + KE * NT * (8 + (int)sizeof(V)));  // KE * NT
+      if (t == 0) s_nh = s_nl = 0;
+      __syncthreads();
+      int lmin = INT_MAX, lmax = -1;
+      long long psum = 0;
+      for (int64_t e = t; e < nA; e += NT) {
+        const int64_t ka = a0 + e;
+        const int L = blen[ka];
+        if (L <= 0) continue;
+        const int64_t b0 = bstart[ka];
+        const int sl = hslot[Aidx[ka]];
+        const int cf = Bidx[b0], cl = Bidx[b0 + L - 1];
+        lmin = min(lmin, cf);
+        lmax = max(lmax, cl);
+        psum += L;
+        if (sl >= 0) {
+          const int h = atomicAdd(&s_nh, 1);
+          hrec[h] = HRec{b0, sl, (int)e};
+        } else {
+          const int l = atomicAdd(&s_nl, 1);
+          lrec[nA - 1 - l] = LRec{0, (int)e, cf, L};
+        }
+      }
+      psum = Group<NT>::reduce(psum, OpAdd(), redl);
+      lmax = Group<NT>::reduce(lmax, OpMax(), red_cnt);
+      lmin = Group<NT>::reduce(lmin, OpMin(), red_cnt);  // the reductions also order the record writes
+      const int nH = s_nh, nL = s_nl;
+      if (nH <= KE * NT && nL <= KE * NT) {  // uniform
+        banded_done = true;
+        int w = lmin == INT_MAX ? 1 : (lmin >> LOGVC);
+        const int wl = lmin == INT_MAX ? 0 : (lmax >> LOGVC);
+        // banded entries k * NT + t: B row start, A value, band row, length, range [s, s1) of the
+        // current window, its end one window on (s1n) and the column after the window (nxc)
+        int64_t hbr[KE];
+        V hv[KE];
+        const int32_t* hbi[KE];
+        int hL[KE], hs[KE], hs1[KE], hs1n[KE], hnx[KE];
+        // other entries: B row start, A value, position, entries left, column at the position
+        int64_t lb0[KE];
+        V lva[KE];
+        int lp[KE], lrem[KE], lnc[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+          const int h = k * NT + t;
+          hL[k] = 0;
+          hs[k] = hs1[k] = hs1n[k] = 0;
+          hnx[k] = INT_MAX;
+          hbr[k] = 0;
+          hv[k] = V(0);
+          hbi[k] = band;
+          if (h < nH) {
+            const HRec r = hrec[h];
+            hbr[k] = r.b0;
+            hv[k] = Aval[a0 + r.e];
+            hbi[k] = band + (int64_t)r.slot * nb1;
+            hL[k] = hbi[k][nb1 - 1];
+            hs[k] = hbi[k][w * BPW];
+            hs1[k] = hbi[k][(w + 1) * BPW];
+          }
+          lrem[k] = 0;
+          lnc[k] = INT_MAX;
+          lp[k] = 0;
+          lb0[k] = 0;
+          lva[k] = V(0);
+          if (h < nL) {
+            const LRec r = lrec[nA - 1 - h];
+            lb0[k] = bstart[a0 + r.e];
+            lva[k] = Aval[a0 + r.e];
+            lrem[k] = r.rem;
+            lnc[k] = r.nc;
+          }
+        }
+        long long windows = 0;  // every window holds an element: at most psum windows
+        while (w <= wl) {
+          const int lo = w << LOGVC;
+          const int64_t hi = (int64_t)lo + VC;
+          // ---- ranges of the banded entries: exclusive scan of their counts ----
+          int csum = 0;
+#pragma unroll
+          for (int k = 0; k < KE; ++k) csum += hs1[k] - hs[k];
+          const int incl = wave_incl_scan_dpp(csum);
+          if (lane == 63) red_cnt[wv] = incl;
+          __syncthreads();
+          int base = 0, T = 0;
+#pragma unroll
+          for (int i = 0; i < NW; ++i) {
+            const int ci = red_cnt[i];
+            base += i < wv ? ci : 0;
+            T += ci;
+          }
+          {
+            int run = base + incl - csum;  // entries k * NT + t: this thread's entries in order
+#pragma unroll
+            for (int k = 0; k < KE; ++k) {
+              hS[t * KE + k] = run;
+              hb0[t * KE + k] = hbr[k] + hs[k];
+              hva[t * KE + k] = hv[k];
+              run += hs1[k] - hs[k];
+            }
+          }
+          // ---- next window's band ends and the columns after this window (loads in flight
+          // with this window's gathers) ----
+#pragma unroll
+          for (int k = 0; k < KE; ++k) {
+            hs1n[k] = hs1[k];
+            hnx[k] = INT_MAX;
+            if (hs1[k] < hL[k]) {
+              hs1n[k] = hbi[k][min((w + 2) * BPW, (int)nb1 - 1)];
+              hnx[k] = Bidx[hbr[k] + hs1[k]];
+            }
+          }
+          // ---- other entries: first four elements of those with one in this window ----
+          int lc[KE][4];
+          V lbv[KE][4];
+#pragma unroll
+          for (int k = 0; k < KE; ++k) {
+            const bool act = lrem[k] > 0 && (int64_t)lnc[k] < hi;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int64_t at = act ? lb0[k] + lp[k] + min(q, lrem[k] - 1) : 0;
+              lc[k][q] = Bidx[at];
+              lbv[k][q] = Bval[at];
+            }
+          }
+          __syncthreads();  // the range table is complete
+          // ---- banded elements: element f belongs to the table entry j with hS[j] <= f < hS[j + 1]
+          // (entries in thread-major order: thread t's KE entries are t * KE .. t * KE + KE - 1) ----
+          constexpr int NE = KE * NT;
+          for (int f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+            int col[U];
+            V bv[U], va[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int f = f0 + u * NT + t;
+              int j = 0;
+#pragma unroll
+              for (int step = NE / 2; step >= 1; step >>= 1)
+                if (hS[j + step] <= f) j += step;  // hS is non-decreasing; entries past nH hold T
+              const int64_t at = f < T ? hb0[j] + (f - hS[j]) : 0;  // lanes past T: unused read
+              col[u] = Bidx[at];
+              bv[u] = Bval[at];
+              va[u] = hva[j];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (f0 + u * NT + t < T) {
+                const int o = col[u] - lo;
+                lds_add(&dv[o], va[u] * bv[u]);
+                atomicOr(&bm[o >> 5], 1u << (o & 31));
+              }
+            }
+          }
+          // ---- other entries: accumulate the loaded elements, continue a run longer than four ----
+#pragma unroll
+          for (int k = 0; k < KE; ++k) {
+            if (lrem[k] > 0 && (int64_t)lnc[k] < hi) {
+              int took = 0;
+              bool stop = false;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                if (!stop) {
+                  if (q < lrem[k] && (int64_t)lc[k][q] < hi) {
+                    const int o = lc[k][q] - lo;
+                    lds_add(&dv[o], lva[k] * lbv[k][q]);
+                    atomicOr(&bm[o >> 5], 1u << (o & 31));
+                    ++took;
+                  } else {
+                    stop = true;
+                  }
+                }
+              }
+              int nc = (took < 4 && took < lrem[k]) ? lc[k][took] : INT_MAX;
+              lp[k] += took;
+              lrem[k] -= took;
+              while (!stop && lrem[k] > 0) {  // rem shrinks every round that continues
+                int cc[4];
+                V bb[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  const int64_t at = lb0[k] + lp[k] + min(q, lrem[k] - 1);
+                  cc[q] = Bidx[at];
+                  bb[q] = Bval[at];
+                }
+                took = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  if (!stop) {
+                    if (q < lrem[k] && (int64_t)cc[q] < hi) {
+                      const int o = cc[q] - lo;
+                      lds_add(&dv[o], lva[k] * bb[q]);
+                      atomicOr(&bm[o >> 5], 1u << (o & 31));
+                      ++took;
+                    } else {
+                      stop = true;
+                      if (q < lrem[k]) nc = cc[q];
+                    }
+                  }
+                }
+                lp[k] += took;
+                lrem[k] -= took;
+              }
+              lnc[k] = lrem[k] > 0 ? nc : INT_MAX;
+            }
+          }
+          // the next window: the smallest next column over all entries
+          int nxt = INT_MAX;
+#pragma unroll
+          for (int k = 0; k < KE; ++k) {
+            nxt = min(nxt, hnx[k]);
+            if (lrem[k] > 0) nxt = min(nxt, lnc[k]);
+          }
+          __syncthreads();  // the window's products are in LDS
+          // emit in column order: one bitmap word per thread, positions by a group scan
+          uint32_t wd = t < OWORD ? bm[t] : 0u;
+          const int cntw = __popc(wd);
+          const int inclw = wave_incl_scan_dpp(cntw);
+          const int wm = wave_reduce_dpp(nxt, OpMin());
+          if (lane == 63) red_cnt[wv] = inclw;
+          if (lane == 0) red[wv] = wm;
+          __syncthreads();
+          int basew = 0, tot = 0, nlo = INT_MAX;
+#pragma unroll
+          for (int i = 0; i < NW; ++i) {
+            const int ci = red_cnt[i];
+            basew += i < wv ? ci : 0;
+            tot += ci;
+            nlo = min(nlo, red[i]);
+          }
+          if (wd) {
+            int64_t pos = out + basew + inclw - cntw;
+            bm[t] = 0u;
+            while (wd) {
+              const int bb = __ffs(wd) - 1;
+              wd &= wd - 1u;
+              const int o = t * 32 + bb;
+              if (pos < c1) {
+                Cidx[pos] = lo + o;
+                Cval[pos] = dv[o];
+              } else {
+                flags |= kErrWindow;
+              }
+              dv[o] = V(0);
+              ++pos;
+            }
+          }
+          out += tot;
+          // every next column lies past this window, so the loop moves on; the banded ranges
+          // advance: a skipped window holds none of their elements, so the next start is this
+          // end, and the next end is the prefetched one unless windows were skipped
+          const int wn = nlo == INT_MAX ? wl + 1 : (nlo >> LOGVC);
+#pragma unroll
+          for (int k = 0; k < KE; ++k) {
+            hs[k] = hs1[k];
+            hs1[k] = (wn == w + 1 || hs1[k] >= hL[k]) ? hs1n[k] : hbi[k][min((wn + 1) * BPW, (int)nb1 - 1)];
+          }
+          w = wn;
+          __syncthreads();  // cleared before the next window's products; reduction slots free
+          if (++windows > psum) {  // uniform: no progress (reported; the row's output is not trusted)
+            flags |= kErrWindow;
+            break;
+          }
+        }
+      }
+      }
+      if (!banded_done) {
       // ================= dense windows of VC columns, one walk =================
       LongState* st = nA <= kOldStates ? ostl : state + a0;
       int lmin = INT_MAX, lmax = -1;
@@ -1880,6 +2274,7 @@ __global__ __launch_bounds__(NT) void k_numeric_long(
           flags |= kErrWindow;
           break;
         }
+      }
       }
     }
     if (out != c1) flags |= kErrWindow;
@@ -2211,7 +2606,8 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int VC = sizeof(V) == 8 ? kNumLongVC64 : kNumLongVC32;
     hipLaunchKernelGGL((k_numeric_long<V, kLongNT, kNumLongBW, VC>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
-                       A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, (LongState*)cursor, err);
+                       A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, (LongState*)cursor, err, A.idx,
+                       B.hslot, B.band, B.nb1);
     return;
   }
 #define X(T_, NT_, G_)                                                                                           \

@@ -127,8 +127,12 @@ const char* dbd_last_error(void);
 void dbd_default_opts(dbd_opts* opts);
 dbd_status dbd_create(const dbd_opts* opts, dbd_ctx** out);
 dbd_status dbd_destroy(dbd_ctx* ctx);
-/* Use a caller-owned hipStream_t (passed as void*) for all device work; NULL -> own stream. */
+/* Use a caller-owned hipStream_t (passed as void*) for all device work. NULL is the HIP
+ * default stream, where torch's default-stream work runs: pass it as is to stay ordered with it.
+ * dbd_use_own_stream returns to a stream the context owns (the state after dbd_create). The
+ * batched form's graph capture needs a stream other than the default one. */
 dbd_status dbd_set_stream(dbd_ctx* ctx, void* hip_stream);
+dbd_status dbd_use_own_stream(dbd_ctx* ctx);
 
 /* ---- operands + plan + run (the reference pipeline) ------------------------------ */
 /* Host views are uploaded (timed as h2d_s); device views are borrowed and must outlive use.

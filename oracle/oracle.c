@@ -235,13 +235,15 @@ static int64_t find_col(const int32_t* cols, int64_t n, int32_t c) {
 /* Masked gradients of C = A * B for an upstream gradient dC on pattern(C):
  *   dA_ik = sum_{j in B_k} dC_ij * B_kj        (on pattern(A))
  *   dB_kj = sum_{i : k in A_i} A_ik * dC_ij    (on pattern(B))
- * Sequential over rows (dB is a scatter), FP64. Returns the number of products whose column
- * was missing from C's row (0 for a C produced by the forward). */
+ * FP64, rows in parallel (OpenMP): dA in B's order within each entry, dB by atomic adds (the
+ * order of the terms of a dB entry varies, within FP64 rounding). Returns the number of products
+ * whose column was missing from C's row (0 for a C produced by the forward). */
 int64_t oracle_backward(int64_t nr, const int64_t* Aptr, const int32_t* Aidx, const double* Aval, const int64_t* Bptr,
                         const int32_t* Bidx, const double* Bval, const int64_t* Cptr, const int32_t* Cidx,
                         const double* dC, double* dA, double* dB, int64_t nnzB) {
   int64_t missing = 0;
   for (int64_t p = 0; p < nnzB; ++p) dB[p] = 0.0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : missing)
   for (int64_t i = 0; i < nr; ++i) {
     const int32_t* cols = Cidx + Cptr[i];
     const int64_t n = Cptr[i + 1] - Cptr[i];
@@ -256,6 +258,7 @@ int64_t oracle_backward(int64_t nr, const int64_t* Aptr, const int32_t* Aidx, co
         }
         const double g = dC[Cptr[i] + m];
         s += g * Bval[b];
+#pragma omp atomic
         dB[b] += Aval[a] * g;
       }
       dA[a] = s;
