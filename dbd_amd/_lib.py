@@ -54,6 +54,14 @@ class dbd_plan_info(C.Structure):
         return d
 
 
+class dbd_table_desc(C.Structure):
+    _fields_ = [("g_small", C.c_int32), ("g_big", C.c_int32), ("load", C.c_double), ("top", C.c_int32),
+                ("hub_min_cols", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class dbd_timing(C.Structure):
     _fields_ = [("preprocess_s", C.c_double), ("compute_gpu_s", C.c_double), ("write_s", C.c_double),
                 ("kernel_exec_s", C.c_double), ("total_s", C.c_double), ("h2d_s", C.c_double),
@@ -92,6 +100,7 @@ SIGNATURES = {
     "dbd_batch_launch": (I32, [P, P, P, P]),
     "dbd_autotune": (I32, [P, C.POINTER(I32)]),
     "dbd_table_count": (I32, []),
+    "dbd_table_knobs": (I32, [I32, C.POINTER(dbd_table_desc)]),
     "dbd_set_table": (I32, [P, I32]),
     "dbd_noalloc_scratch_bytes": (C.c_size_t, [I64, I64]),
     "dbd_spgemm_noalloc": (I32, [CSRP, CSRP, P, P, P, P, I64, C.POINTER(I64)]),

@@ -11,7 +11,7 @@ from typing import Optional
 import numpy as np
 
 from ._lib import (DEVICE, F32, F64, HOST, MODE_COMPUTE, MODE_D2H, MODE_IO, check, dbd_csr, dbd_opts, dbd_plan_info,
-                   dbd_timing, lib)
+                   dbd_table_desc, dbd_timing, lib)
 
 _MODES = {"compute": MODE_COMPUTE, "d2h": MODE_D2H, "io": MODE_IO, "sspmm": MODE_IO,
           MODE_COMPUTE: MODE_COMPUTE, MODE_D2H: MODE_D2H, MODE_IO: MODE_IO}
@@ -318,6 +318,15 @@ class Context:
     @staticmethod
     def table_count() -> int:
         return int(lib.dbd_table_count())
+
+    @staticmethod
+    def table_knobs(table_id: int) -> dict:
+        """Knobs of candidate table `table_id` (table 0 is the default): lanes per B row in small /
+        large hash tables, the load limit, the largest numeric LDS table, and the column count from
+        which hub rows take the numeric hub kernels."""
+        d = dbd_table_desc()
+        check(lib.dbd_table_knobs(table_id, C.byref(d)))
+        return d.as_dict()
 
     def set_table(self, table_id: int) -> None:
         check(lib.dbd_set_table(self._h, table_id))

@@ -96,14 +96,19 @@ struct HostBuf {
 // Default table. Symbolic key: min(P_i, ncols) (a bound on nnz_i); numeric key: nnz_i.
 // Hash tables are sized so the load stays <= `load`; the one-wave symbolic tables (keys only,
 // 4 bytes a slot, up to 1024 slots) to half of that, which shortens their probe sequences.
-Tables dbd::default_tables(int g_small, int g_big, double load) {
+Tables dbd::default_tables(int g_small, int g_big, double load, int top, int64_t hub_min_cols) {
   Tables t;
+  t.g_small = g_small;
+  t.g_big = g_big;
+  t.load = load;
+  t.top = top;
+  t.hub_min_cols = hub_min_cols;
   const int Ts[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
   const int NTs[] = {64, 64, 64, 64, 128, 256, 512, 1024, 1024};
   int n = 0;
   t.sym_ct.lim[n] = 0;
   t.sym_k[n++] = {0, 0, 0};
-  for (int i = 0; i < 9; ++i) {
+  for (int i = 0; i < 9 && Ts[i] <= 2 * top; ++i) {
     t.sym_ct.lim[n] = (int64_t)(Ts[i] * (Ts[i] <= 1024 ? load * 0.5 : load));
     t.sym_k[n++] = {Ts[i], Ts[i] <= 1024 ? 64 : NTs[i], Ts[i] <= 1024 ? g_small : g_big};
   }
@@ -113,32 +118,58 @@ Tables dbd::default_tables(int g_small, int g_big, double load) {
   n = 0;
   t.num_ct.lim[n] = 0;
   t.num_k[n++] = {0, 0, 0};
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < 8 && Ts[i] <= top; ++i) {
     t.num_ct.lim[n] = (int64_t)(Ts[i] * load);
     t.num_k[n++] = {Ts[i], NTs[i], Ts[i] <= 1024 ? g_small : g_big};
   }
   t.num_ct.lim[n] = INT64_MAX;
-  t.num_k[n++] = {0, 256, 16};
+  t.num_k[n++] = {0, 256, 16};  // long rows
+  t.num_ct.lim[n] = INT64_MAX;  // hub rows: long rows of more than kHubEntries A entries
+  t.num_k[n++] = {kHubShape, 1024, 0};
   t.num_ct.n = n;
   return t;
 }
 
 namespace {
 
+// The autotuner's search space: every combination of five knobs, each listed default first, so
+// table 0 is the default table and id = mixed-radix digits (g_small, g_big, load, top, hub).
+constexpr int kKnobG[2] = {16, 64};
+constexpr int kKnobGBig[2] = {64, 16};
+constexpr double kKnobLoad[2] = {0.5, 0.25};
+constexpr int kKnobTop[2] = {8192, 4096};
+constexpr int64_t kKnobHub[3] = {kHubAutoCols, 0, INT64_MAX};  // by width, always, never
+constexpr int kKnobRadix[5] = {2, 2, 2, 2, 3};
+
 std::vector<Tables>& candidate_tables() {
   static std::vector<Tables> c = [] {
     std::vector<Tables> v;
-    int id = 0;
-    for (int gs : {16, 64})
-      for (int gb : {16, 64})
-        for (double load : {0.5, 0.25}) {
-          Tables t = default_tables(gs, gb, load);
-          t.id = id++;
-          v.push_back(t);
-        }
+    for (int id = 0; id < 2 * 2 * 2 * 2 * 3; ++id) {
+      int d[5], r = id;
+      for (int k = 4; k >= 0; --k) {
+        d[k] = r % kKnobRadix[k];
+        r /= kKnobRadix[k];
+      }
+      Tables t = default_tables(kKnobG[d[0]], kKnobGBig[d[1]], kKnobLoad[d[2]], kKnobTop[d[3]], kKnobHub[d[4]]);
+      t.id = id;
+      v.push_back(t);
+    }
     return v;
   }();
   return c;
+}
+
+// id of the table whose knob k is set to value index x, the other knobs as in table `id`
+int table_with_knob(int id, int k, int x) {
+  int d[5], r = id;
+  for (int j = 4; j >= 0; --j) {
+    d[j] = r % kKnobRadix[j];
+    r /= kKnobRadix[j];
+  }
+  d[k] = x;
+  int out = 0;
+  for (int j = 0; j < 5; ++j) out = out * kKnobRadix[j] + d[j];
+  return out;
 }
 
 // autotuner cache: problem class -> table id
@@ -279,6 +310,10 @@ struct dbd_ctx {
   // band index of B's long rows (for the long-row window kernels), built once per operand set
   DevBuf band_hist, band_flag, band_tmp, band_slot, band_row, band;
   bool band_ready = false;
+  // workspace of the hub-row kernels (zero between launches)
+  DevBuf hubpool;
+  HubPool hp;
+  bool hub_dirty = false;
   int64_t band_minlen = 0, band_rows = 0, band_nb1 = 0;
   std::vector<int64_t> h_blk, h_blockP, h_blockC, h_offs_num, h_sym_tot;
   dbd_plan_info info{};
@@ -318,8 +353,33 @@ dbd_status check_err(dbd_ctx* c, const char* phase) {
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  if (e != 0)
+  if (e != 0) {
+    c->hub_dirty = true;  // a kernel stopped early: the hub pool may hold values
     return set_error(DBD_ERR_INTERNAL, "%s: device consistency check failed (flags 0x%x)", phase, e);
+  }
+  return DBD_OK;
+}
+
+// Hub-row pool: one slice of ncols presence bytes and ncols values per workgroup of the hub
+// kernels' persistent grid, as many slices as fit min(4 GiB, device budget / 8), at most one per
+// compute unit. Zeroed when allocated, and again after a failed launch (the kernels leave it zero).
+dbd_status ensure_hub(dbd_ctx* c) {
+  const int64_t stride = (c->B.ncols + 255) / 256 * 256;
+  const int64_t slice = stride * (1 + (int64_t)c->vsz);
+  if (!c->hubpool.p || c->hp.stride != stride || c->hp.slice_bytes != slice) {
+    const int64_t cap = std::min<int64_t>((int64_t)4 << 30, (c->o.device_budget_mib << 20) / 8);
+    const int64_t n = std::max<int64_t>(1, std::min<int64_t>(c->num_cu, cap / slice));
+    CHK(c->hubpool.ensure((size_t)(n * slice)));
+    c->hp.base = c->hubpool.as<uint8_t>();
+    c->hp.stride = stride;
+    c->hp.slice_bytes = slice;
+    c->hp.slices = (int)n;
+    c->hub_dirty = true;
+  }
+  if (c->hub_dirty) {
+    launch_fill_bytes0(c->hubpool.p, (int64_t)c->hp.slices * c->hp.slice_bytes, c->st);
+    c->hub_dirty = false;
+  }
   return DBD_OK;
 }
 
@@ -472,10 +532,11 @@ dbd_status symbolic(dbd_ctx* c) {
   // numeric bins by exact nnz_i
   const CatTable& nt = c->tb.num_ct;
   const int64_t n_off = (int64_t)nt.n * c->nwg;
-  launch_bin_count(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s);
+  const int64_t* hub_rows = hub_enabled(c->tb, c->B.ncols) ? c->A.ptr : nullptr;
+  launch_bin_count(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s, hub_rows);
   launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_num.as<int64_t>(), n_off, c->tmp.as<int64_t>(), s);
   launch_bin_scatter(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->offs_num.as<int64_t>(), c->rows_num.as<int32_t>(),
-                     s);
+                     s, hub_rows);
   c->h_offs_num.assign((size_t)n_off + 1, 0);
   HIPCHK(hipMemcpyAsync(c->h_offs_num.data(), c->offs_num.p, 8 * (size_t)(n_off + 1), hipMemcpyDeviceToHost, s));
   CHK(check_err(c, "symbolic"));  // synchronises the stream
@@ -504,8 +565,9 @@ dbd_status numeric_blocks(dbd_ctx* c, int64_t b0, int64_t b1, const int64_t* Cpt
     const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
     if (n <= 0) continue;
     if (c->tb.num_k[k].T == 0) CHK(ensure_cursor(c));
+    if (c->tb.num_k[k].T == kHubShape) CHK(ensure_hub(c));
     launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
-                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st, distinct);
+                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st, distinct, c->hp);
   }
   return DBD_OK;
 }
@@ -543,7 +605,7 @@ int64_t fixed_device_bytes(dbd_ctx* c) {
   s += (int64_t)c->cnt.bytes + c->offs_sym.bytes + c->offs_num.bytes + c->tmp.bytes + c->cursor.bytes;
   s += (int64_t)c->bdesc_s.bytes + c->bdesc_l.bytes;
   s += (int64_t)c->band_hist.bytes + c->band_flag.bytes + c->band_tmp.bytes + c->band_slot.bytes + c->band_row.bytes +
-       c->band.bytes;
+       c->band.bytes + c->hubpool.bytes;
   return s;
 }
 
@@ -824,7 +886,7 @@ dbd_status dbd_create(const dbd_opts* opts, dbd_ctx** out) {
   dbd_ctx* c = new dbd_ctx();
   c->o = o;
   c->vsz = dtype_size(o.dtype);
-  c->tb = default_tables(16, 64, 0.5);
+  c->tb = candidate_tables()[0];
   if (hipGetDevice(&c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -843,7 +905,7 @@ dbd_status dbd_destroy(dbd_ctx* c) {
   if (c->cst) (void)hipStreamSynchronize(c->cst);
   for (auto* b : {&c->P, &c->Pptr, &c->nnz, &c->Cptr, &c->rows_sym, &c->rows_num, &c->cnt, &c->offs_sym, &c->offs_num,
                   &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng, &c->bdesc_s, &c->bdesc_l, &c->band_hist,
-                  &c->band_flag, &c->band_tmp, &c->band_slot, &c->band_row, &c->band})
+                  &c->band_flag, &c->band_tmp, &c->band_slot, &c->band_row, &c->band, &c->hubpool})
     b->release();
   for (int i = 0; i < 3; ++i) {
     c->upA[i].release();
@@ -1042,7 +1104,8 @@ dbd_status dbd_backward(dbd_ctx* c, const int64_t* d_Cptr, const int32_t* d_Cidx
     const int64_t i0 = (int64_t)k * c->nwg, i1 = (int64_t)(k + 1) * c->nwg;
     const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
     if (n <= 0) continue;
-    if (c->tb.num_k[k].T == 0) CHK(ensure_cursor(c));
+    // long and hub rows both take the window backward kernel, which keeps a cursor per A entry
+    if (c->tb.num_k[k].T == 0 || c->tb.num_k[k].T == kHubShape) CHK(ensure_cursor(c));
     launch_backward(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
                     d_Cptr, d_Cidx, d_dC, d_dA, d_dB, c->cursor.as<int64_t>(), c->err.as<int>(), s);
   }
@@ -1062,7 +1125,10 @@ static int persistent_grid(int num_cu, const KernelShape& k) {
 // synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
 // the numeric kernels drain empty ranges and the capacity bit is set in the error word.
 void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu,
-                             int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s) {
+                             int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s,
+                             const HubPool& hub) {
+  // without a pool, or on problems the table keeps off the hub kernels, hub rows stay long
+  const int64_t* hub_rows = hub.slices > 0 && hub_enabled(tb, B.ncols) ? A.ptr : nullptr;
   const int64_t nr = w.nr;
   A.bstart = w.bstart;
   A.blen = w.blen;
@@ -1077,13 +1143,16 @@ void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B,
                     persistent_grid(num_cu, tb.sym_k[k]), A, B, w.nnz, w.cursor, w.err, s);
   launch_exclusive_scan(w.nnz, d_Cptr, nr, w.tmp, s);
   const CatTable& nt = tb.num_ct;
-  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s);
+  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s, hub_rows);
   launch_exclusive_scan(w.cnt, w.offs_num, (int64_t)nt.n * w.nwg, w.tmp, s);
-  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s);
+  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s, hub_rows);
   launch_capacity_guard(d_Cptr + nr, cap, w.offs_num, (int64_t)nt.n * w.nwg + 1, w.err, s);
-  for (int k = 1; k < nt.n; ++k)
+  for (int k = 1; k < nt.n; ++k) {
+    if (tb.num_k[k].T == kHubShape && !hub_rows) continue;
     launch_numeric(dtype, tb.num_k[k], w.rows_num, w.offs_num + (int64_t)k * w.nwg, w.offs_num + (int64_t)(k + 1) * w.nwg,
-                   0, persistent_grid(num_cu, tb.num_k[k]), A, B, d_Cptr, 0, d_Cidx, d_Cval, w.cursor, w.err, s);
+                   0, persistent_grid(num_cu, tb.num_k[k]), A, B, d_Cptr, 0, d_Cidx, d_Cval, w.cursor, w.err, s, false,
+                   hub);
+  }
 }
 
 // The batch's pipeline on the context's workspace. C was sized by the symbolic pass of
@@ -1105,7 +1174,7 @@ static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int3
   w.err = c->err.as<int>();
   w.nr = c->nr;
   w.nwg = c->nwg;
-  enqueue_pipeline(w, c->tb, c->A, c->B, c->o.dtype, c->num_cu, d_Cptr, d_Cidx, d_Cval, c->batch_cap, s);
+  enqueue_pipeline(w, c->tb, c->A, c->B, c->o.dtype, c->num_cu, d_Cptr, d_Cidx, d_Cval, c->batch_cap, s, c->hp);
   return DBD_OK;
 }
 
@@ -1150,11 +1219,16 @@ dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* 
     HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
     memcpy(c->g_key, key, sizeof(key));
   }
+  if (c->hub_dirty && c->hubpool.p) {  // the graph's hub kernels expect a clear pool
+    launch_fill_bytes0(c->hubpool.p, (int64_t)c->hp.slices * c->hp.slice_bytes, c->st);
+    c->hub_dirty = false;
+  }
   HIPCHK(hipGraphLaunch(c->gexec, c->st));
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->symb = false;
+  if (e != 0) c->hub_dirty = true;
   if (e & kErrCapacityBit)
     return set_error(DBD_ERR_BUDGET, "batch: C needs more than the %lld entries dbd_batch_prepare reported "
                      "(operand structure changed since); nothing was written", (long long)c->batch_cap);
@@ -1166,6 +1240,20 @@ dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* 
 
 extern "C" {
 int32_t dbd_table_count(void) { return (int32_t)candidate_tables().size(); }
+
+dbd_status dbd_table_knobs(int32_t id, dbd_table_desc* out) {
+  clear_error();
+  if (!out) return set_error(DBD_ERR_INVALID, "null argument");
+  if (id < 0 || id >= (int32_t)candidate_tables().size())
+    return set_error(DBD_ERR_INVALID, "table id %d out of range [0, %d)", id, (int)candidate_tables().size());
+  const Tables& t = candidate_tables()[(size_t)id];
+  out->g_small = t.g_small;
+  out->g_big = t.g_big;
+  out->load = t.load;
+  out->top = t.top;
+  out->hub_min_cols = t.hub_min_cols;
+  return DBD_OK;
+}
 
 dbd_status dbd_set_table(dbd_ctx* c, int32_t id) {
   clear_error();
@@ -1196,12 +1284,18 @@ dbd_status dbd_autotune(dbd_ctx* c, int32_t* chosen) {
     if (chosen) *chosen = hit;
     return DBD_OK;
   }
-  // time symbolic + numeric for each candidate on the real operands
-  int best = 0;
-  double best_ms = 1e30;
+  // Coordinate descent over the knobs from the default table: each knob in turn takes the value
+  // whose table runs symbolic + numeric fastest on the real operands, the others held at the best
+  // so far (1 + 1 + 1 + 1 + 1 + 2 = 7 tables timed instead of all 48).
   DevBuf ci, cv;
-  for (const Tables& cand : candidate_tables()) {
-    c->tb = cand;
+  std::map<int, double> timed;
+  auto time_table = [&](int id, double* ms_out) -> dbd_status {
+    auto it = timed.find(id);
+    if (it != timed.end()) {
+      *ms_out = it->second;
+      return DBD_OK;
+    }
+    c->tb = candidate_tables()[(size_t)id];
     CHK(plan(c));
     double tot = 0;
     for (int rep = 0; rep < 2; ++rep) {
@@ -1217,9 +1311,23 @@ dbd_status dbd_autotune(dbd_ctx* c, int32_t* chosen) {
       if (rep > 0) tot += ms;
     }
     CHK(check_err(c, "autotune"));
-    if (tot < best_ms) {
-      best_ms = tot;
-      best = cand.id;
+    timed[id] = tot;
+    *ms_out = tot;
+    return DBD_OK;
+  };
+  int best = 0;
+  double best_ms = 0;
+  CHK(time_table(best, &best_ms));
+  for (int k = 0; k < 5; ++k) {
+    const int base = best;
+    for (int x = 0; x < kKnobRadix[k]; ++x) {
+      const int id = table_with_knob(base, k, x);
+      double ms = 0;
+      CHK(time_table(id, &ms));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = id;
+      }
     }
   }
   ci.release();

@@ -37,9 +37,23 @@ struct Mat {
 
 // One category's kernel shape, chosen by the planner/autotuner.
 struct KernelShape {
-  int T;   // LDS hash slots (power of two); 0 = long-row window kernel
+  int T;   // LDS hash slots (power of two); 0 = long-row window kernel; kHubShape = hub-row kernel
   int NT;  // threads per row group
   int G;   // lanes cooperating on one A entry (B row)
+};
+constexpr int kHubShape = -1;
+// Rows with more A entries than this, among the rows of the long (last by key) category, form the
+// hub category: one workgroup per row streams every B row once into a dense slice of the pool.
+constexpr int64_t kHubEntries = 512;
+
+// Workspace of the hub-row kernels: `slices` slices of `slice_bytes`, one per workgroup of their
+// persistent grid; a slice holds `stride` presence bytes (stride >= ncols, a multiple of 256), then
+// stride values. Zero when handed to a kernel; the kernels leave it zero.
+struct HubPool {
+  uint8_t* base = nullptr;
+  int64_t stride = 0;
+  int64_t slice_bytes = 0;
+  int slices = 0;
 };
 
 // ---- generic device utilities ----
@@ -83,10 +97,12 @@ void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, i
 // Stable counting sort of rows [0, n) by category of min(key[i], cap).
 // counts/offs have ncat * nwg (+1) int64 where nwg = bin_tiles(n).
 int64_t bin_tiles(int64_t n);
+// With Aptr, rows of the category before the last (the long one by key) that have more than
+// kHubEntries A entries go to the last category (the hub category).
 void launch_bin_count(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, int64_t* counts,
-                      hipStream_t s);
+                      hipStream_t s, const int64_t* Aptr = nullptr);
 void launch_bin_scatter(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, const int64_t* offs,
-                        int32_t* rows, hipStream_t s);
+                        int32_t* rows, hipStream_t s, const int64_t* Aptr = nullptr);
 
 // ---- symbolic / numeric / backward per category ----
 // grid == 0 -> exact grid from nrows_hint; else a persistent grid of that many workgroups.
@@ -100,7 +116,7 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
 void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                     const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                     const int64_t* Cptr, int64_t cbase, int32_t* Cidx, void* Cval, int64_t* cursor, int* err,
-                    hipStream_t s, bool distinct = false);
+                    hipStream_t s, bool distinct = false, const HubPool& hub = HubPool{});
 void launch_backward(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                      const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                      const int64_t* Cptr, const int32_t* Cidx, const void* dC, void* dA, void* dB, int64_t* cursor,

@@ -895,8 +895,19 @@ __device__ __forceinline__ int category_of(int64_t k, const CatTable& ct) {
   return ct.n - 1;
 }
 
+// Category of row i: by key. Tables end with the long category (limit INT64_MAX, so no key goes
+// past it) and the hub category; with Aptr, a long row with more than kHubEntries A entries is a
+// hub row. Without Aptr the hub category stays empty.
+__device__ __forceinline__ int row_category(int64_t k, const CatTable& ct, const int64_t* __restrict__ Aptr,
+                                            int64_t i) {
+  const int c = category_of(k, ct);
+  if (Aptr && c == ct.n - 2 && Aptr[i + 1] - Aptr[i] > kHubEntries) return ct.n - 1;
+  return c;
+}
+
 __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restrict__ key, int64_t n, int64_t cap,
-                                                        CatTable ct, int64_t* __restrict__ counts, int64_t nwg) {
+                                                        CatTable ct, int64_t* __restrict__ counts, int64_t nwg,
+                                                        const int64_t* __restrict__ Aptr) {
   __shared__ int hist[kMaxCats];
   if (threadIdx.x < kMaxCats) hist[threadIdx.x] = 0;
   __syncthreads();
@@ -904,7 +915,7 @@ __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restric
   if (i < n) {
     int64_t k = key[i];
     if (k > cap) k = cap;
-    atomicAdd(&hist[category_of(k, ct)], 1);
+    atomicAdd(&hist[row_category(k, ct, Aptr, i)], 1);
   }
   __syncthreads();
   if (threadIdx.x < ct.n) counts[(int64_t)threadIdx.x * nwg + blockIdx.x] = hist[threadIdx.x];
@@ -912,14 +923,14 @@ __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restric
 
 __global__ __launch_bounds__(kBinTile) void k_bin_scatter(const int64_t* __restrict__ key, int64_t n, int64_t cap,
                                                           CatTable ct, const int64_t* __restrict__ offs, int64_t nwg,
-                                                          int32_t* __restrict__ rows) {
+                                                          int32_t* __restrict__ rows, const int64_t* __restrict__ Aptr) {
   __shared__ int wc[kBinTile / 64][kMaxCats];
   const int64_t i = (int64_t)blockIdx.x * kBinTile + threadIdx.x;
   int c = -1;
   if (i < n) {
     int64_t k = key[i];
     if (k > cap) k = cap;
-    c = category_of(k, ct);
+    c = row_category(k, ct, Aptr, i);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -937,16 +948,17 @@ __global__ __launch_bounds__(kBinTile) void k_bin_scatter(const int64_t* __restr
 }
 
 void launch_bin_count(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, int64_t* counts,
-                      hipStream_t s) {
+                      hipStream_t s, const int64_t* Aptr) {
   const int64_t nwg = bin_tiles(n);
   if (nwg == 0) return;
-  hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, counts, nwg);
+  hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, counts, nwg, Aptr);
 }
 void launch_bin_scatter(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, const int64_t* offs,
-                        int32_t* rows, hipStream_t s) {
+                        int32_t* rows, hipStream_t s, const int64_t* Aptr) {
   const int64_t nwg = bin_tiles(n);
   if (nwg == 0) return;
-  hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, offs, nwg, rows);
+  hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, offs, nwg, rows,
+                     Aptr);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1162,6 +1174,170 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
     if (t == 0) nnz[row] = count;
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------------------------------------
+// hub rows: rows with more A entries than the window kernels hold (kHubEntries)
+// ------------------------------------------------------------------------------------
+// A numeric window kernel visits every A entry of its row in every dense window of 4096 columns
+// it sweeps, so a row's cost grows as nA * ncols / 4096 beside its products: over a few million
+// columns a row of thousands of entries spends more on those visits than on its products. A hub
+// row instead streams each entry's B row once: the products of all its entries are gathered
+// element-parallel over a 1024-thread workgroup (entry ranges flattened by a scan, chunks of NT
+// entries) and accumulated into the workgroup's own presence bytes and dense values in a workspace
+// pool (one slice of `stride` columns per workgroup of a persistent grid), then one sweep over the
+// row's column span emits the columns in order and clears what was set. Its price is a global
+// atomic per product, which is why only wide problems route rows here (the table's hub_min_cols).
+// Each slice is touched by one workgroup only; a device-scope fence orders the accumulation
+// before the sweep. The symbolic pass counts hub rows with the bitmap windows, whose 131072-column
+// windows make the visits cheap.
+template <typename V, int NT>
+__device__ __forceinline__ void hub_accumulate(int t, int64_t a0, int64_t nA, const int64_t* __restrict__ bstart,
+                                               const int32_t* __restrict__ blen, const V* __restrict__ Aval,
+                                               const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+                                               uint8_t* __restrict__ pres, V* __restrict__ dval, int64_t* hb0,
+                                               V* hva, int64_t* hS, long long* scl) {
+  constexpr int U = 4;
+  for (int64_t e0 = 0; e0 < nA; e0 += NT) {  // uniform: chunks of NT entries
+    long long cnt = 0;
+    if (e0 + t < nA) {
+      const int64_t ka = a0 + e0 + t;
+      cnt = blen[ka];
+      hb0[t] = bstart[ka];
+      hva[t] = Aval[ka];
+    }
+    long long T;  // elements of the chunk (64-bit: a chunk of long B rows can pass 2^31)
+    hS[t] = Group<NT>::exscan(cnt, T, scl);
+    const int nch = (int)min((int64_t)NT, nA - e0);
+    __syncthreads();
+    for (int64_t f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+      int col[U];
+      V bv[U], va[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t f = f0 + u * NT + t;
+        int j = 0;  // largest j with hS[j] <= f: the chunk entry whose B row holds element f
+#pragma unroll
+        for (int step = NT / 2; step >= 1; step >>= 1)
+          if (j + step < nch && hS[j + step] <= f) j += step;
+        const int64_t at = f < T ? hb0[j] + (f - hS[j]) : 0;  // lanes past T: unused read
+        col[u] = Bidx[at];
+        bv[u] = Bval[at];
+        va[u] = hva[j];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (f0 + u * NT + t < T) {
+          pres[col[u]] = 1;
+          atomicAdd(&dval[col[u]], va[u] * bv[u]);
+        }
+    }
+    __syncthreads();  // the chunk tables are rewritten by the next chunk
+  }
+}
+
+// Column span [lo, hi] of a row: first and last B column over its entries (B rows are sorted).
+template <int NT>
+__device__ __forceinline__ void hub_span(int t, int64_t a0, int64_t nA, const int64_t* __restrict__ bstart,
+                                         const int32_t* __restrict__ blen, const int32_t* __restrict__ Bidx, int* sc,
+                                         int& lo, int& hi) {
+  int l = INT_MAX, h = -1;
+  for (int64_t e = t; e < nA; e += NT) {
+    const int64_t ka = a0 + e;
+    const int L = blen[ka];
+    if (L > 0) {
+      l = min(l, Bidx[bstart[ka]]);
+      h = max(h, Bidx[bstart[ka] + L - 1]);
+    }
+  }
+  lo = Group<NT>::reduce(l, OpMin(), sc);
+  hi = Group<NT>::reduce(h, OpMax(), sc);
+}
+
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void k_numeric_hub(const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg,
+                                                    const int64_t* __restrict__ pend, const int64_t* __restrict__ Aptr,
+                                                    const int64_t* __restrict__ bstart,
+                                                    const int32_t* __restrict__ blen, const V* __restrict__ Aval,
+                                                    const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+                                                    const int64_t* __restrict__ Cptr, int64_t cbase,
+                                                    int32_t* __restrict__ Cidx, V* __restrict__ Cval,
+                                                    uint8_t* __restrict__ pool, int64_t stride,
+                                                    int64_t slice_bytes, int* __restrict__ err) {
+  constexpr int NW = NT / 64;
+  __shared__ int64_t hb0[NT];
+  __shared__ V hva[NT];
+  __shared__ int64_t hS[NT];
+  __shared__ int red_cnt[NW];
+  __shared__ int sc[NW];
+  __shared__ long long scl[NW];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint8_t* pres = pool + (int64_t)blockIdx.x * slice_bytes;
+  V* dval = reinterpret_cast<V*>(pres + stride);
+  int flags = 0;
+  const int64_t beg = *pbeg, end = *pend;
+  for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
+    const int row = rows[slot];
+    const int64_t a0 = Aptr[row], nA = Aptr[row + 1] - a0;
+    const int64_t c0 = Cptr[row] - cbase, c1 = Cptr[row + 1] - cbase;
+    int lo, hi;
+    hub_span<NT>(t, a0, nA, bstart, blen, Bidx, sc, lo, hi);
+    hub_accumulate<V, NT>(t, a0, nA, bstart, blen, Aval, Bidx, Bval, pres, dval, hb0, hva, hS, scl);
+    __threadfence();
+    __syncthreads();
+    // sweep [lo, hi] in rounds of 16 columns per thread: present columns in order, at positions
+    // from a group scan of the per-thread counts; values read and cleared with the bytes
+    int64_t out = c0;
+    if (hi >= lo) {
+      const int64_t q0 = lo >> 4, q1 = hi >> 4;
+      uint4* pw = reinterpret_cast<uint4*>(pres);
+      for (int64_t qb = q0; qb <= q1; qb += NT) {  // uniform
+        const int64_t q = qb + t;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (q <= q1) v = pw[q];
+        const int cnt = __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+        const int incl = wave_incl_scan_dpp(cnt);
+        if (lane == 63) red_cnt[wv] = incl;
+        __syncthreads();
+        int base = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+          const int ci = red_cnt[i];
+          base += i < wv ? ci : 0;
+          tot += ci;
+        }
+        if (cnt) {
+          int64_t pos = out + base + incl - cnt;
+          const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            uint32_t bits = w4[k];  // one byte per column, 0 or 1
+            while (bits) {
+              const int b = (__ffs(bits) - 1) >> 3;
+              bits &= ~(0xFFu << (8 * b));
+              const int64_t col = q * 16 + k * 4 + b;
+              if (pos < c1) {
+                Cidx[pos] = (int)col;
+                Cval[pos] = dval[col];
+              } else {
+                flags |= kErrWindow;
+              }
+              dval[col] = V(0);
+              ++pos;
+            }
+          }
+          pw[q] = make_uint4(0, 0, 0, 0);
+        }
+        out += tot;
+        __syncthreads();  // reduction slots free for the next round
+      }
+    }
+    if (out != c1) flags |= kErrWindow;
+    __threadfence();
+    __syncthreads();  // cleared before the next row of this workgroup
+  }
+  flags = wave_reduce_dpp(flags, OpOr());
+  if (lane == 0 && flags) atomicOr(err, flags);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2564,15 +2740,20 @@ constexpr int kLongG = 16;
   X(8192, 1024, G)
 
 int symbolic_groups_per_wg(const KernelShape& k) {
-  if (k.T == 0) return 1;
+  if (k.T <= 0) return 1;  // long and hub rows: one workgroup per row
   return k.NT <= 64 ? 256 / k.NT : 1;
+}
+
+static int hub_grid(int64_t nrows_hint, int grid, const HubPool& hub) {
+  if (grid > 0) return hub.slices;  // persistent launches drain any number of rows
+  return (int)std::max<int64_t>(1, std::min<int64_t>(nrows_hint, hub.slices));
 }
 
 void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                      int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
                      int* err, hipStream_t s) {
   const int gpb = symbolic_groups_per_wg(k);
-  if (k.T == 0) {
+  if (k.T == 0 || k.T == kHubShape) {  // the symbolic pass counts hub rows with the long-row windows
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     hipLaunchKernelGGL((k_symbolic_long<1024, kSymLongW, kLongG>), gr, dim3(1024), 0, s, rows, pbeg, pend, A.ptr,
                        A.bstart, A.blen, B.idx, nnz, cursor, err);
@@ -2598,10 +2779,16 @@ template <typename V>
 static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                              int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr,
                              int64_t cbase, int32_t* Cidx, V* Cval, int64_t* cursor, int* err, hipStream_t s,
-                             bool distinct) {
+                             bool distinct, const HubPool& hub) {
   const int gpb = symbolic_groups_per_wg(k);
   const V* Av = (const V*)A.val;
   const V* Bv = (const V*)B.val;
+  if (k.T == kHubShape) {
+    hipLaunchKernelGGL((k_numeric_hub<V, 1024>), dim3(hub_grid(nrows_hint, grid, hub)), dim3(1024), 0, s, rows, pbeg,
+                       pend, A.ptr, A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, hub.base, hub.stride,
+                       hub.slice_bytes, err);
+    return;
+  }
   if (k.T == 0) {
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int VC = sizeof(V) == 8 ? kNumLongVC64 : kNumLongVC32;
@@ -2636,13 +2823,14 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
 
 void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                     int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr, int64_t cbase,
-                    int32_t* Cidx, void* Cval, int64_t* cursor, int* err, hipStream_t s, bool distinct) {
+                    int32_t* Cidx, void* Cval, int64_t* cursor, int* err, hipStream_t s, bool distinct,
+                    const HubPool& hub) {
   if (dtype == 1)
     launch_numeric_t<double>(k, rows, pbeg, pend, nrows_hint, grid, A, B, Cptr, cbase, Cidx, (double*)Cval, cursor,
-                             err, s, distinct);
+                             err, s, distinct, hub);
   else
     launch_numeric_t<float>(k, rows, pbeg, pend, nrows_hint, grid, A, B, Cptr, cbase, Cidx, (float*)Cval, cursor,
-                            err, s, distinct);
+                            err, s, distinct, hub);
 }
 
 template <typename V>
@@ -2653,7 +2841,7 @@ static void launch_backward_t(const KernelShape& k, const int32_t* rows, const i
   const int gpb = symbolic_groups_per_wg(k);
   const V* Av = (const V*)A.val;
   const V* Bv = (const V*)B.val;
-  if (k.T == 0) {
+  if (k.T == 0 || k.T == kHubShape) {  // long and hub rows: windows over the C row
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int W = sizeof(V) == 8 ? kBwdLongW64 : kBwdLongW32;
     hipLaunchKernelGGL((k_backward_long<V, kLongNT, W, kLongG>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
@@ -2685,7 +2873,7 @@ void launch_backward(int dtype, const KernelShape& k, const int32_t* rows, const
 }
 
 bool shape_available(int phase, const KernelShape& k) {
-  if (k.T == 0) return true;
+  if (k.T == 0 || k.T == kHubShape) return true;
   if (k.G != 16 && k.G != 64) return false;
   if (k.G > k.NT) return false;
   const int T = k.T, NT = k.NT;

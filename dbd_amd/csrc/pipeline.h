@@ -22,16 +22,31 @@
 
 namespace dbd {
 
-// A category table: thresholds and kernel shapes of the symbolic and numeric categories.
+// Problems of at least this many columns route hub rows to the hub kernels by default.
+constexpr int64_t kHubAutoCols = (int64_t)1 << 21;
+
+// A category table: thresholds and kernel shapes of the symbolic and numeric categories, and the
+// knobs they were built from (the autotuner's search space).
 struct Tables {
   CatTable sym_ct, num_ct;
   KernelShape sym_k[kMaxCats], num_k[kMaxCats];
   int id = 0;
+  int g_small = 16, g_big = 64;  // lanes per A entry in tables of <= 1024 / larger slot counts
+  double load = 0.5;             // hash-table load limit
+  int top = 8192;                // slots of the largest numeric LDS table (symbolic: twice that)
+  int64_t hub_min_cols = kHubAutoCols;  // numeric hub category used when ncols >= this
 };
 
 // Symbolic key min(P_i, ncols), numeric key nnz_i; hash tables sized so the load stays <= `load`
-// (one-wave symbolic tables at half of that); G lanes per A entry for small / large tables.
-Tables default_tables(int g_small, int g_big, double load);
+// (one-wave symbolic tables at half of that); G lanes per A entry for small / large tables; the
+// numeric tables stop at `top` slots (4096 or 8192), the symbolic ones at 2 * top. Rows past the
+// last table are long rows; the numeric table ends with the hub category, used when B has at
+// least hub_min_cols columns.
+Tables default_tables(int g_small, int g_big, double load, int top = 8192, int64_t hub_min_cols = kHubAutoCols);
+
+// Whether the numeric binning routes hub rows (long rows of more than kHubEntries A entries) to
+// the hub category for a problem of ncols columns.
+inline bool hub_enabled(const Tables& t, int64_t ncols) { return ncols >= t.hub_min_cols; }
 
 // Raw device workspace of one pipeline run: per-row arrays (nr + 1), bin counters and offsets
 // (kMaxCats per 256-row tile), scan partials, per-A-nonzero B-row descriptors and long-row
@@ -47,7 +62,9 @@ struct Work {
 // persistent category kernels that read their row ranges from device memory: no host
 // synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
 // the numeric kernels drain empty ranges and the capacity bit is set in the error word.
+// Hub rows take the hub kernels with `hub`'s pool when the table enables them for B's width, and
+// stay in the long category when the pool is empty.
 void enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu, int64_t* d_Cptr,
-                      int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s);
+                      int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s, const HubPool& hub = HubPool{});
 
 }  // namespace dbd

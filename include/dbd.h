@@ -178,12 +178,26 @@ dbd_status dbd_backward(dbd_ctx* ctx, const int64_t* d_Cptr, const int32_t* d_Ci
  * is returned. */
 dbd_status dbd_batch_prepare(dbd_ctx* ctx, const dbd_csr* A_cat, const dbd_csr* B_cat, int64_t* nnz_out);
 dbd_status dbd_batch_launch(dbd_ctx* ctx, int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval);
-/* Autotuner: time candidate category tables on the operands set in ctx and keep the fastest
- * for this problem class (cached in-process). Returns the chosen table's id. */
+/* Autotuner: search the candidate category tables on the operands set in ctx (coordinate
+ * descent over the knobs of dbd_table_desc from table 0, timing symbolic + numeric of each table
+ * it visits) and keep the fastest for this problem class (cached in-process). Returns the chosen
+ * table's id. */
 dbd_status dbd_autotune(dbd_ctx* ctx, int32_t* chosen);
-/* Candidate category tables (thresholds, hash-table sizes, lanes per B row) the autotuner
- * chooses from; dbd_set_table selects one explicitly (results do not depend on the choice). */
+/* The knobs a candidate table is built from. Table 0 is the default table. */
+typedef struct dbd_table_desc {
+  int32_t g_small;      /* lanes per B row in hash tables of up to 1024 slots (16 or 64) */
+  int32_t g_big;        /* lanes per B row in larger tables (16 or 64) */
+  double load;          /* hash-table load limit: a table of T slots takes rows of <= load*T keys */
+  int32_t top;          /* slots of the largest numeric LDS table (4096 or 8192; symbolic 2x);
+                           rows beyond it take the long-row windows */
+  int64_t hub_min_cols; /* long rows of more than 512 A entries take the numeric hub kernels when
+                           B has at least this many columns (2^21: by width; 0: always;
+                           INT64_MAX: never) */
+} dbd_table_desc;
+/* Candidate category tables (every combination of the knobs above: 48) the autotuner chooses
+ * from; dbd_set_table selects one explicitly (results do not depend on the choice). */
 int32_t dbd_table_count(void);
+dbd_status dbd_table_knobs(int32_t table_id, dbd_table_desc* out);
 dbd_status dbd_set_table(dbd_ctx* ctx, int32_t table_id);
 
 /* ---- entry point that allocates nothing ------------------------------------------- */

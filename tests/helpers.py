@@ -32,9 +32,12 @@ def compare(C, ref, dtype, what=""):
                             f"{float(np.max(err / np.maximum(Cabs, 1e-300))):.3e}")
 
 
-def gpu_spgemm(A, B=None, mode="compute", block_rows=4096, **kw):
-    """Full pipeline run; returns (host CSR, timing, plan info)."""
+def gpu_spgemm(A, B=None, mode="compute", block_rows=4096, table=None, **kw):
+    """Full pipeline run (with candidate table `table` when given); returns (host CSR, timing,
+    plan info)."""
     with dbd_amd.Context(block_rows=block_rows, dtype=A.val.dtype, **kw) as ctx:
+        if table is not None:
+            ctx.set_table(table)
         ctx.set_operands(A, B)
         t = ctx.run(mode)
         info = ctx.plan_info()

@@ -93,18 +93,28 @@ def _skewed_matrix(n_rows, ncols, long_rows, seed, zipf=1.6):
     return sp.csr_matrix((v, (r, c)), shape=(n_rows, ncols))
 
 
+@pytest.mark.parametrize("table", [0, 1])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_all_categories(gpu_ok, dtype):
+def test_all_categories(gpu_ok, dtype, table):
     """Row lengths from 1 to tens of thousands: every hash-table size and both window paths
-    (symbolic key min(P_i, ncols) above 8192, numeric nnz_i above 4096) get rows."""
+    (symbolic key min(P_i, ncols) above 8192, numeric nnz_i above 4096) get rows. Table 0 (the
+    default) keeps this 60,000-column problem's hub rows in the long category; table 1 routes them
+    to the numeric hub kernels."""
+    assert dbd_amd.Context.table_knobs(1)["hub_min_cols"] == 0
     n, ncols = 6000, 60000
     long_rows = {17: 3000, 900: 5000, 4500: 1200, 5999: 40}
     A = _skewed_matrix(n, n, long_rows, 11)
     B = _skewed_matrix(n, ncols, {3: 2500, 4: 20000, 100: 800}, 12, zipf=1.4)
     Ac, Bc = csr_sp(A, dtype), csr_sp(B, dtype)
-    C, t, info = check_product(Ac, Bc, what="all-categories", block_rows=512)
-    assert info["sym_rows"][-1] > 0, info["sym_rows"]
-    assert info["num_rows"][-1] > 0, info["num_rows"]
+    C, t, info = check_product(Ac, Bc, what=f"all-categories, table {table}", block_rows=512, table=table)
+    # the symbolic pass's last category is the long rows; the numeric pass's the hub rows (long
+    # rows of more than 512 A entries, rows 17, 900 and 4500 among them), the one before it the
+    # other long rows
+    assert info["sym_rows"][-1] >= 4, info["sym_rows"]
+    if table == 1:
+        assert info["num_rows"][-1] >= 3, info["num_rows"]
+    else:
+        assert info["num_rows"][-1] == 0 and info["num_rows"][-2] >= 4, info["num_rows"]
     assert sum(1 for x in info["num_rows"][1:] if x > 0) >= 6, info["num_rows"]
 
 
@@ -129,7 +139,9 @@ def test_long_rows_wide_column_range(gpu_ok, dtype):
     A = sp.csr_matrix((rng.standard_normal(sum(x.size for x in cols_a)),
                        (np.concatenate(rows_a), np.concatenate(cols_a))), shape=(50, 64))
     C, t, info = check_product(csr_sp(A, dtype), csr_sp(B, dtype), what="wide long rows", block_rows=512)
-    assert info["num_rows"][-1] == 50 and info["sym_rows"][-1] == 50, (info["num_rows"], info["sym_rows"])
+    # long rows (numeric: the category before the hub rows), none with more than 512 A entries
+    assert info["num_rows"][-2] == 50 and info["sym_rows"][-1] == 50, (info["num_rows"], info["sym_rows"])
+    assert info["num_rows"][-1] == 0
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -156,7 +168,7 @@ def test_long_rows_dense_column_band(gpu_ok, dtype):
     A = sp.csr_matrix((rng.standard_normal(sum(x.size for x in cols_a)),
                        (np.concatenate(rows_a), np.concatenate(cols_a))), shape=(30, 40))
     C, t, info = check_product(csr_sp(A, dtype), csr_sp(B, dtype), what="dense band", block_rows=512)
-    assert info["num_rows"][-1] == 30, info["num_rows"]
+    assert info["num_rows"][-2] == 30, info["num_rows"]
     assert int(np.diff(C.ptr).min()) > 8192
 
 
