@@ -170,7 +170,7 @@ def main():
     ap.add_argument("--io-dir", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=5)
-    ap.add_argument("--workload", default="er2m", choices=["er2m", "web", "rmat22", "batch", "fwdbwd"],
+    ap.add_argument("--workload", default="er2m", choices=["er2m", "web", "rmat22", "batch", "fwdbwd", "rocsparse"],
                     help="er2m (default headline) or one of the other named configurations")
     ap.add_argument("--batch-members", type=int, default=512)
     ap.add_argument("--rmat-scale", type=int, default=22)

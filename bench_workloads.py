@@ -10,6 +10,9 @@
             nnz/row U[4, 64] with Zipf(1.8) row skew): one captured graph vs per-matrix plain runs
     fwdbwd  R-MAT scale 18 A (row-normalised adjacency + self loops) times an independent R-MAT
             B, FP32, through torch.autograd: forward vs forward + backward
+    rocsparse  rocSPARSE SpGEMM (whole matrix and per 512-row block) beside DBD's kernel window
+            on the web-like, Erdos-Renyi 2M and R-MAT 20 slice products, DBD's result checked
+            against rocSPARSE's
 
 Each prints one JSON line on stdout.
 """
@@ -163,6 +166,37 @@ def rmat_kernel_window(scale=22, steps=1, warmup=1, A=None):
     log(f"rmat{scale}: kernel window {t * 1e3:.1f} ms ({2 * P / t / 1e9:.1f} GFLOPS), numeric pass "
         f"{num_s * 1e3:.1f} ms = {nb / num_s / 1e9:.0f} GB/s")
     return res, A
+
+
+def rocsparse_run(args, timeout=1200):
+    """Run the rocSPARSE harness program (dbd_amd/bin/rocsparse_spgemm) and return its JSON line.
+    It times DBD's kernel window and rocSPARSE's SpGEMM (whole matrix, and per 512-row block as
+    the reference's block harness runs it) on the same operands in its own process, and checks
+    DBD's product against rocSPARSE's. A program of its own because a process that imports torch
+    binds torch's bundled rocSPARSE and HIP runtime; its progress lines go to stderr as they come."""
+    import subprocess
+    exe = os.path.join(ROOT, "dbd_amd", "bin", "rocsparse_spgemm")
+    r = subprocess.run([exe, *[str(x) for x in args]], stdout=subprocess.PIPE, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"rocsparse_spgemm {' '.join(map(str, args))} exited with {r.returncode}; "
+                           f"stdout: {r.stdout[-2000:]!r}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def rocsparse_comparisons():
+    """rocSPARSE beside DBD on the web-like 281,903^2 (FP64, DBD 4096-row blocks), the Erdos-Renyi
+    2M^2 x 16 (FP64, this benchmark's workload) and an R-MAT scale 20 slice (rows 0, 16, 32, ... of
+    A times A, FP64)."""
+    import torch
+    torch.cuda.empty_cache()  # the harness allocates in its own process
+    res = {}
+    log("rocSPARSE beside DBD: web-like 281,903^2")
+    res["web"] = rocsparse_run(["--web", 281903, 1, "--dbd-block-rows", 4096])
+    log("rocSPARSE beside DBD: Erdos-Renyi 2,000,000^2 x 16")
+    res["er2m"] = rocsparse_run(["--er", 2_000_000, 2_000_000, 2_000_000, 16, 16, 1, 2, "--dbd-block-rows", 512])
+    log("rocSPARSE beside DBD: R-MAT scale 20 slice")
+    res["rmat20_slice"] = rocsparse_run(["--rmat-slice", 20, 16, "--dbd-block-rows", 512])
+    return res
 
 
 def run_rmat22(args):
@@ -345,3 +379,7 @@ def run_fwdbwd(args):
     res = forward_backward(max(1, args.steps), args.warmup)
     res["metric"] = "forward+backward / forward time"
     print(json.dumps(res))
+
+
+def run_rocsparse(args):
+    print(json.dumps(rocsparse_comparisons()), flush=True)

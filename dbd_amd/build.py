@@ -68,6 +68,10 @@ def build(verbose: bool = True, lib_out: str | None = None, defines: list[str] |
     os.replace(tmp, lib)
     if not cli:
         return lib
+    # rocSPARSE comparison harness (a program of its own: bench and tests only)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", os.path.join(CSRC, "vendor_spgemm.cpp"), "-o",
+          os.path.join(BIN, "rocsparse_spgemm"), "-L", HERE, "-ldbd", "-Wl,-rpath,$ORIGIN/..", "-L", "/opt/rocm/lib",
+          "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"])
     modes = {"compute": ["-DDBD_CLI_MODE=0"], "d2h": ["-DDBD_CLI_MODE=1"],
              "SSpMM": ["-DDBD_CLI_MODE=2"], "debug": ["-DDBD_CLI_MODE=2", "-DDBD_CLI_DEBUG"]}
     with ThreadPoolExecutor(max_workers=4) as ex:
@@ -78,7 +82,7 @@ def build(verbose: bool = True, lib_out: str | None = None, defines: list[str] |
         for f in futs:
             f.result()
     if verbose:
-        print(f"built {LIB} and {BIN}/{{compute,d2h,SSpMM,debug}} for {ARCH}")
+        print(f"built {LIB} and {BIN}/{{compute,d2h,SSpMM,debug,rocsparse_spgemm}} for {ARCH}")
     return LIB
 
 
