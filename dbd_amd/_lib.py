@@ -34,7 +34,8 @@ class dbd_opts(C.Structure):
     _fields_ = [("block_rows", C.c_int32), ("dtype", C.c_int32), ("host_budget_mib", C.c_int64),
                 ("device_budget_mib", C.c_int64), ("tune", C.c_int32), ("verbose", C.c_int32),
                 ("out_dir", C.c_char_p), ("ptr64_file", C.c_int32), ("writer_threads", C.c_int32),
-                ("chunks_hint", C.c_int32), ("retain_result", C.c_int32)]
+                ("chunks_hint", C.c_int32), ("retain_result", C.c_int32), ("hybrid", C.c_int32),
+                ("cpu_threads", C.c_int32)]
 
 
 class dbd_plan_info(C.Structure):
@@ -66,7 +67,8 @@ class dbd_timing(C.Structure):
     _fields_ = [("preprocess_s", C.c_double), ("compute_gpu_s", C.c_double), ("write_s", C.c_double),
                 ("kernel_exec_s", C.c_double), ("total_s", C.c_double), ("h2d_s", C.c_double),
                 ("d2h_s", C.c_double), ("d2h_bytes", C.c_double), ("io_bytes", C.c_double),
-                ("chunks", C.c_int64), ("symbolic_s", C.c_double), ("numeric_s", C.c_double)]
+                ("chunks", C.c_int64), ("symbolic_s", C.c_double), ("numeric_s", C.c_double),
+                ("cpu_s", C.c_double), ("cpu_blocks", C.c_int64), ("cpu_products", C.c_int64)]
 
     def to_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -133,7 +135,7 @@ def _load() -> C.CDLL:
         f = getattr(lb, name)
         f.restype = res
         f.argtypes = args
-    if lb.dbd_abi_version() != 2:
+    if lb.dbd_abi_version() != 3:
         raise ImportError("libdbd ABI version mismatch")
     return lb
 

@@ -196,11 +196,14 @@ def analyze_block(criteria: int, avg_products_per_row: int) -> int:
 # ---- the pipeline -----------------------------------------------------------------------
 class Context:
     """One dbd_ctx: operands, plan, runs. Mirrors the reference's binaries' knobs: block rows
-    (ROWS_IN_BLOCK), host and device DRAM budgets in GiB, and the three modes."""
+    (ROWS_IN_BLOCK), host and device DRAM budgets in GiB, and the three modes. hybrid=True runs
+    the blocks the reference's analyze_Block sends to the CPU on host threads (D2H and IO modes;
+    the operands must be host CSR)."""
 
     def __init__(self, block_rows: int = 4096, dtype=np.float64, host_gib: float = 28, device_gib: float = 22,
                  tune: bool = False, out_dir: Optional[str] = None, ptr64: bool = False, writer_threads: int = 4,
-                 chunks_hint: int = 8, retain: bool = True, stream=None):
+                 chunks_hint: int = 8, retain: bool = True, stream=None, hybrid: bool = False,
+                 cpu_threads: int = 0):
         o = dbd_opts()
         lib.dbd_default_opts(C.byref(o))
         o.block_rows = block_rows
@@ -214,6 +217,8 @@ class Context:
         o.writer_threads = writer_threads
         o.chunks_hint = chunks_hint
         o.retain_result = int(retain)
+        o.hybrid = int(hybrid)
+        o.cpu_threads = int(cpu_threads)
         self.dtype = o.dtype
         h = C.c_void_p()
         check(lib.dbd_create(C.byref(o), C.byref(h)))

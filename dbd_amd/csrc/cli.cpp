@@ -13,6 +13,8 @@
 //   --tune           autotune category tables for this input
 //   --writers N      file writer threads
 //   --no-cache       do not read or write the <input>.csr cache
+//   --hybrid         CPU co-execution: CPU-preferred blocks on host threads (d2h, SSpMM)
+//   --cpu-threads N  host threads of --hybrid (default: all)
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -65,6 +67,10 @@ int main(int argc, char** argv) {
       o.writer_threads = atoi(argv[++i]);
     } else if (a == "--no-cache") {
       use_cache = 0;
+    } else if (a == "--hybrid") {
+      o.hybrid = 1;
+    } else if (a == "--cpu-threads" && i + 1 < argc) {
+      o.cpu_threads = atoi(argv[++i]);
     } else if (a.size() > 2 && a.compare(0, 2, "--") == 0) {
       usage();
     } else {
@@ -122,21 +128,26 @@ int main(int argc, char** argv) {
   if (dbd_get_plan_info(ctx, &pi) != DBD_OK) return fail("plan info");
   printf("------------ AVG TIME ------------\n");
   printf("Preprocessing\t\t%f s\n", t.preprocess_s);
-  printf("Compute CPU\t\t%f s\n", 0.0);
+  printf("Compute CPU\t\t%f s\n", t.cpu_s);
   printf("Compute GPU\t\t%f s\n", t.compute_gpu_s);
   printf("Write File\t\t%f s\n", t.write_s);
   printf("Kernel Execution\t%f s\n", t.kernel_exec_s);
   printf("Total Execution\t\t%f s\n", t.total_s);
-  printf(" - Workload Distribution -\nCPUs\tBlocks:   %d (%.2f %%)\n\tElements: %d (%.2f %%)\n", 0, 0.0, 0, 0.0);
-  printf("GPUs\tBlocks:   %lld (%.2f %%)\n\tElements: %lld (%.2f %%)\n\n", (long long)pi.num_blocks, 100.0,
-         (long long)pi.products, 100.0);
+  const long long nbk = (long long)pi.num_blocks, np_ = (long long)pi.products;
+  const long long cb = (long long)t.cpu_blocks, cpr = (long long)t.cpu_products;
+  printf(" - Workload Distribution -\nCPUs\tBlocks:   %lld (%.2f %%)\n\tElements: %lld (%.2f %%)\n", cb,
+         nbk ? 100.0 * (double)cb / (double)nbk : 0.0, cpr, np_ ? 100.0 * (double)cpr / (double)np_ : 0.0);
+  printf("GPUs\tBlocks:   %lld (%.2f %%)\n\tElements: %lld (%.2f %%)\n\n", nbk - cb,
+         nbk ? 100.0 * (double)(nbk - cb) / (double)nbk : 0.0, np_ - cpr,
+         np_ ? 100.0 * (double)(np_ - cpr) / (double)np_ : 0.0);
 #ifdef DBD_CLI_DEBUG
   // Z/main.cpp:110-116. Blocks reach the files one chunk per write (each chunk is a run of
   // consecutive blocks written with one pwrite per file, the I/O-binding writer's coalescing,
   // Z/writeFile_binding.cpp:60-168), so the blocks merged into another block's write are
   // num_blocks - chunks.
   puts("\n - DEBUG -");
-  const long long merged = t.chunks > 0 ? (long long)pi.num_blocks - (long long)t.chunks : 0;
+  // (host blocks of the hybrid mode are written one block per write)
+  const long long merged = t.chunks > 0 ? (long long)pi.num_blocks - (long long)t.cpu_blocks - (long long)t.chunks : 0;
   printf("Merged %lld Blocks while Writing (%f)\n", merged,
          pi.num_blocks > 0 ? (double)merged / (double)pi.num_blocks : 0.0);
   printf("Computed Total %lld Elements (%lld Bytes = %lld MebiBytes)\n", (long long)pi.products,

@@ -23,6 +23,19 @@ void clear_error();
 // dbd_plan_blocks share.
 void block_criteria(const int64_t* block_products, int64_t nb, int32_t block_rows, dbd_plan_info* info);
 
+// Host multiply of the hybrid mode: rows [r0, r1) of C = A*B (host CSR views of one dtype) into
+// idx/val at positions Cptr[r] - base, columns in increasing order (Gustavson's row-wise method
+// with a dense accumulator and a touched-column list, the scheme of compute_CPU,
+// Z/computeCPU.cpp:10-98, with the first touch decided by structure, not by |acc| < 1e-4).
+// Returns false if a row's column count differs from Cptr's (the GPU's symbolic count).
+struct HostRowScratch {
+  std::vector<double> acc;      // ncols
+  std::vector<int64_t> stamp;   // ncols: last row that touched the column, -1 never
+  std::vector<int32_t> cols;    // touched columns of the current row
+};
+bool host_rows(const dbd_csr& A, const dbd_csr& B, int64_t r0, int64_t r1, const int64_t* Cptr, int64_t base,
+               int32_t* idx, void* val, HostRowScratch& s);
+
 inline int hw_threads() {
   unsigned n = std::thread::hardware_concurrency();
   if (n == 0) n = 1;

@@ -22,6 +22,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -178,12 +179,13 @@ std::map<std::string, int> g_tune_cache;
 
 // ---- writer pool for DBD_MODE_IO ---------------------------------------------------------
 struct WriteJob {
-  int slot;
-  hipEvent_t ready;
+  int slot;          // pinned ring slot, or -1 for a host buffer of the hybrid mode's CPU blocks
+  hipEvent_t ready;  // the slot's copy is done (nullptr: nothing to wait for)
   const void* idx;
   const void* val;
   int64_t n;
-  int64_t off;  // element offset of the chunk in C
+  int64_t off;                // element offset of the chunk in C
+  std::shared_ptr<void> keep;  // owner of a host buffer, released when both files have it
 };
 
 // File writers of DBD_MODE_IO: one thread per output file (Idx.output, Val.output), each
@@ -221,7 +223,7 @@ struct Writer {
         q[f].pop_front();
       }
       double t0 = now_s();
-      bool ok = hipEventSynchronize(j.ready) == hipSuccess;
+      bool ok = j.ready == nullptr || hipEventSynchronize(j.ready) == hipSuccess;
       const size_t es = f == 0 ? 4 : vsz;
       ok = ok && pw(f == 0 ? fd_idx : fd_val, f == 0 ? j.idx : j.val, es * (size_t)j.n, (off_t)(es * (size_t)j.off));
       double t1 = now_s();
@@ -233,8 +235,9 @@ struct Writer {
           failed = true;
           why = strerror(errno);
         }
-        --slot_busy[j.slot];
+        if (j.slot >= 0) --slot_busy[j.slot];
       }
+      j.keep.reset();
       cv_free.notify_all();
     }
   }
@@ -299,6 +302,8 @@ struct dbd_ctx {
   DevBuf upA[3], upB[3];
   int64_t a_bytes = 0, b_bytes = 0;
   double h2d_s = 0;
+  dbd_csr hostA{}, hostB{};  // the caller's host views (the hybrid mode's CPU blocks read them)
+  bool host_ops = false;
 
   // plan state
   Tables tb;
@@ -577,12 +582,19 @@ struct Chunk {
 };
 
 // Group consecutive blocks so each chunk's output stays within `cap` elements.
-dbd_status make_chunks(dbd_ctx* c, int64_t cap, std::vector<Chunk>& out) {
+// Chunks: runs of whole consecutive blocks whose C fits `cap` elements. Blocks marked in `cpu`
+// (the hybrid mode's host blocks) are left out: chunks end before them and resume after.
+dbd_status make_chunks(dbd_ctx* c, int64_t cap, std::vector<Chunk>& out, const std::vector<char>* cpu = nullptr) {
   out.clear();
   int64_t b = 0, e = 0;
   while (b < c->nb) {
+    if (cpu && (*cpu)[(size_t)b]) {
+      e += c->h_blockC[(size_t)b++];
+      continue;
+    }
     Chunk ch{b, b, e, e};
     while (ch.b1 < c->nb) {
+      if (cpu && (*cpu)[(size_t)ch.b1]) break;
       const int64_t add = c->h_blockC[(size_t)ch.b1];
       if (ch.b1 > ch.b0 && (ch.e1 - ch.e0) + add > cap) break;
       if (add > cap)
@@ -638,6 +650,22 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
   if (avail <= 0)
     return set_error(DBD_ERR_BUDGET, "Not Enough Memory for Dense Matrix for Result: device budget %lld MiB",
                      (long long)c->o.device_budget_mib);
+  // hybrid mode: the blocks analyze_Block sends to the CPU (outcome 1) go to host threads
+  const bool hybrid = c->o.hybrid && mode != DBD_MODE_COMPUTE;
+  if (hybrid && !c->host_ops)
+    return set_error(DBD_ERR_INVALID, "hybrid mode needs host operand views (its host threads read A and B)");
+  std::vector<char> cpu_blk;
+  std::vector<int64_t> cpu_list;
+  int64_t cpu_prod = 0;
+  if (hybrid) {
+    cpu_blk.assign((size_t)c->nb, 0);
+    for (int64_t b = 0; b < c->nb; ++b)
+      if (dbd_analyze_block(c->info.criteria, c->h_blockP[(size_t)b] / c->o.block_rows) == 1) {
+        cpu_blk[(size_t)b] = 1;
+        cpu_list.push_back(b);
+        cpu_prod += c->h_blockP[(size_t)b];
+      }
+  }
   std::vector<Chunk> chunks;
   const bool whole_dev = (mode == DBD_MODE_COMPUTE) && (int64_t)(nnzC * esz) <= avail;
   int64_t cap;
@@ -654,7 +682,7 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
     for (int64_t v : c->h_blockC) mxb = std::max(mxb, v);
     cap = std::max(cap, std::min(mxb, avail / (int64_t)(2 * esz)));
   }
-  CHK(make_chunks(c, cap, chunks));
+  CHK(make_chunks(c, cap, chunks, hybrid ? &cpu_blk : nullptr));
   int64_t maxch = 1;
   for (auto& ch : chunks) maxch = std::max(maxch, ch.e1 - ch.e0);
   const int nbuf = (whole_dev || mode == DBD_MODE_COMPUTE) ? 1 : 2;
@@ -707,6 +735,63 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
       wr.start(nslots);
     }
   }
+  // hybrid: host threads take the CPU blocks off a shared counter while the GPU runs its chunks;
+  // each writes its rows at their final offsets (Cptr from the symbolic pass): into the host
+  // result, or a buffer handed to the file writers, or (a D2H run that keeps no host result) a
+  // scratch buffer that is dropped like the GPU chunks' ring slots
+  std::vector<std::thread> cpu_th;
+  std::atomic<int64_t> cpu_next{0};
+  std::atomic<bool> cpu_bad{false}, cpu_stop{false};
+  std::atomic<int64_t> cpu_done_ns{0};
+  struct CpuJoin {
+    std::vector<std::thread>& th;
+    std::atomic<bool>& stop;
+    ~CpuJoin() {
+      stop = true;
+      for (auto& x : th)
+        if (x.joinable()) x.join();
+    }
+  } cpu_join{cpu_th, cpu_stop};
+  const double cpu_t0 = now_s();
+  if (hybrid && !cpu_list.empty()) {
+    HIPCHK(hipStreamSynchronize(c->st));  // h_cptr is read by the host threads
+    const int64_t BR = c->o.block_rows;
+    const int64_t* cp = c->h_cptr.as<int64_t>();
+    struct HostPart {
+      std::vector<int32_t> idx;
+      std::vector<char> val;
+    };
+    auto worker = [&, BR, cp]() {
+      HostRowScratch sc;
+      HostPart scratch;
+      for (;;) {
+        const int64_t q = cpu_next.fetch_add(1);
+        if (q >= (int64_t)cpu_list.size() || cpu_stop || cpu_bad) break;
+        const int64_t b = cpu_list[(size_t)q];
+        const int64_t r0 = b * BR, r1 = std::min(c->nr, r0 + BR);
+        const int64_t e0 = cp[r0], n = cp[r1] - e0;
+        bool ok;
+        if (whole_host) {
+          ok = host_rows(c->hostA, c->hostB, r0, r1, cp, 0, c->h_idx.as<int32_t>(), c->h_val.p, sc);
+        } else {
+          auto part = std::make_shared<HostPart>();
+          HostPart& hp = mode == DBD_MODE_IO ? *part : scratch;
+          hp.idx.resize((size_t)std::max<int64_t>(n, 1));
+          hp.val.resize(c->vsz * (size_t)std::max<int64_t>(n, 1));
+          ok = host_rows(c->hostA, c->hostB, r0, r1, cp, e0, hp.idx.data(), hp.val.data(), sc);
+          if (ok && mode == DBD_MODE_IO && n > 0) wr.push({-1, nullptr, hp.idx.data(), hp.val.data(), n, e0, part});
+        }
+        if (!ok) cpu_bad = true;
+      }
+      const int64_t ns = (int64_t)((now_s() - cpu_t0) * 1e9);
+      int64_t prev = cpu_done_ns.load();
+      while (prev < ns && !cpu_done_ns.compare_exchange_weak(prev, ns)) {
+      }
+    };
+    const int nt = (int)std::min<int64_t>(c->o.cpu_threads > 0 ? c->o.cpu_threads : hw_threads(),
+                                          (int64_t)cpu_list.size());
+    for (int w = 0; w < nt; ++w) cpu_th.emplace_back(worker);
+  }
   // chunk loop
   double comp_ms = 0, copy_ms = 0;
   for (size_t i = 0; i < chunks.size(); ++i) {
@@ -746,6 +831,9 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
     }
   }
   HIPCHK(hipEventRecord(c->ev_t1, s));
+  for (auto& x : cpu_th) x.join();  // the host blocks are pushed to the writers before they stop
+  cpu_th.clear();
+  if (cpu_bad) return set_error(DBD_ERR_INTERNAL, "hybrid mode: a host row's column count differs from the symbolic pass");
   HIPCHK(hipStreamSynchronize(c->cst));
   HIPCHK(hipStreamSynchronize(s));
   if (mode == DBD_MODE_IO) {
@@ -796,6 +884,10 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
   t.total_s = t2 - t0;
   t.chunks = (int64_t)chunks.size();
   t.d2h_s = 0;
+  t.cpu_s = cpu_done_ns.load() * 1e-9;
+  t.cpu_blocks = (int64_t)cpu_list.size();
+  t.cpu_products = cpu_prod;
+  (void)cpu_t0;
   c->dev_result = whole_dev && c->o.retain_result && mode == DBD_MODE_COMPUTE;
   c->host_result = whole_host;
   if (tm) *tm = t;
@@ -981,6 +1073,9 @@ dbd_status dbd_set_operands(dbd_ctx* c, const dbd_csr* A, const dbd_csr* B) {
   }
   HIPCHK(hipStreamSynchronize(c->st));
   c->h2d_s = (A->location == DBD_HOST || B->location == DBD_HOST) ? now_s() - t0 : 0.0;
+  c->host_ops = A->location == DBD_HOST && B->location == DBD_HOST;
+  c->hostA = *A;
+  c->hostB = *B;
   c->ops = true;
   if (c->gexec) {
     (void)hipGraphExecDestroy(c->gexec);

@@ -84,6 +84,45 @@ void block_criteria(const int64_t* block_products, int64_t nb, int32_t block_row
   info->blocks_any = a;
 }
 
+bool host_rows(const dbd_csr& A, const dbd_csr& B, int64_t r0, int64_t r1, const int64_t* Cptr, int64_t base,
+               int32_t* idx, void* val, HostRowScratch& s) {
+  if ((int64_t)s.acc.size() < B.ncols) {
+    s.acc.assign((size_t)B.ncols, 0.0);
+    s.stamp.assign((size_t)B.ncols, -1);
+  }
+  const bool f64 = A.dtype == DBD_F64;
+  for (int64_t r = r0; r < r1; ++r) {
+    s.cols.clear();
+    for (int64_t ka = A.ptr[r]; ka < A.ptr[r + 1]; ++ka) {
+      const int32_t k = A.idx[ka];
+      const double a = f64 ? static_cast<const double*>(A.val)[ka] : (double)static_cast<const float*>(A.val)[ka];
+      for (int64_t kb = B.ptr[k]; kb < B.ptr[k + 1]; ++kb) {
+        const int32_t j = B.idx[kb];
+        const double b = f64 ? static_cast<const double*>(B.val)[kb] : (double)static_cast<const float*>(B.val)[kb];
+        if (s.stamp[(size_t)j] != r) {
+          s.stamp[(size_t)j] = r;
+          s.acc[(size_t)j] = a * b;
+          s.cols.push_back(j);
+        } else {
+          s.acc[(size_t)j] += a * b;
+        }
+      }
+    }
+    const int64_t o = Cptr[r] - base;
+    if ((int64_t)s.cols.size() != Cptr[r + 1] - Cptr[r]) return false;
+    std::sort(s.cols.begin(), s.cols.end());
+    for (size_t q = 0; q < s.cols.size(); ++q) {
+      const int32_t j = s.cols[q];
+      idx[o + (int64_t)q] = j;
+      if (f64)
+        static_cast<double*>(val)[o + (int64_t)q] = s.acc[(size_t)j];
+      else
+        static_cast<float*>(val)[o + (int64_t)q] = (float)s.acc[(size_t)j];
+    }
+  }
+  return true;
+}
+
 }  // namespace dbd
 
 extern "C" {

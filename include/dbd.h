@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define DBD_ABI_VERSION 2
+#define DBD_ABI_VERSION 3
 
 typedef enum dbd_status {
   DBD_OK = 0,
@@ -80,6 +80,12 @@ typedef struct dbd_opts {
                                 slots (0 -> 4); each output file has one writer thread */
   int32_t chunks_hint;       /* minimum number of output chunks in D2H/IO modes; 0 -> 8 */
   int32_t retain_result;     /* 1: keep whole C (device in COMPUTE mode, host in D2H mode) if it fits */
+  int32_t hybrid;            /* 1: CPU co-execution in D2H and IO modes (Z/distributeWorkload.cpp:69-144):
+                                blocks whose analyze_Block outcome is 1 (CPU preferred) are multiplied
+                                by host threads into the host result or the files, the GPU does the
+                                rest. Needs host operand views that stay valid during dbd_run.
+                                Ignored in COMPUTE mode (C is kept on the device there). 0: off */
+  int32_t cpu_threads;       /* host threads of the hybrid mode (0 -> the hardware's thread count) */
 } dbd_opts;
 
 #define DBD_MAX_CATS 16
@@ -117,6 +123,9 @@ typedef struct dbd_timing {
   int64_t chunks;       /* output chunks used */
   double symbolic_s;    /* device time of the symbolic pass (row counts, scan, numeric bins) */
   double numeric_s;     /* device time from the end of the symbolic pass to the last numeric chunk */
+  double cpu_s;         /* "Compute CPU": wall time of the hybrid mode's host threads */
+  int64_t cpu_blocks;   /* blocks the host threads computed (hybrid mode) */
+  int64_t cpu_products; /* their products (the CPU's share of the reference's "Elements") */
 } dbd_timing;
 
 typedef struct dbd_ctx dbd_ctx;

@@ -40,7 +40,7 @@ def test_python_signatures_cover_header():
 
 
 def test_abi_version_and_errors_without_gpu():
-    assert _lib.lib.dbd_abi_version() == 2
+    assert _lib.lib.dbd_abi_version() == 3
     import torch
     if torch.cuda.is_available():
         pytest.skip("this check is for hosts without a GPU")

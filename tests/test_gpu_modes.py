@@ -168,3 +168,81 @@ def test_two_phase_device_api_with_torch(gpu_ok):
         C = dbd_amd.CSR(Cptr.cpu().numpy(), Cidx.cpu().numpy(), Cval.cpu().numpy(), A.shape)
     compare(C, ref, np.float64, "two-phase")
     assert info["products"] == int(np.diff(B.ptr)[A.idx].sum())
+
+
+def _light_then_heavy(dtype=np.float64):
+    """A of 10,240 light rows (2 entries: 16 products each) then 10,240 heavy rows (16 entries: 128
+    products) times B (8 entries per row). With 512-row blocks the block averages are 16 and 128;
+    the reference criteria (mean + floor(1.28 sigma) = 143) make the light blocks CPU-preferred
+    (analyze_Block 1: average <= 64) and the heavy ones 'any' (2)."""
+    k, n = 30000, 40000
+    A1 = dbd_amd.gen_er(10240, k, 2, seed=41, dtype=dtype)
+    A2 = dbd_amd.gen_er(10240, k, 16, seed=42, dtype=dtype)
+    ptr = np.concatenate([A1.ptr, A1.ptr[-1] + A2.ptr[1:]])
+    A = dbd_amd.CSR(ptr, np.concatenate([A1.idx, A2.idx]), np.concatenate([A1.val, A2.val]), (20480, k))
+    B = dbd_amd.gen_er(k, n, 8, seed=43, dtype=dtype)
+    return A, B
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_hybrid_d2h_cpu_blocks_match_oracle(gpu_ok, dtype):
+    """Hybrid mode (Z/distributeWorkload.cpp:69-144): the CPU-preferred blocks are multiplied by host
+    threads into the host result, the GPU does the rest; the whole C equals the oracle's and the
+    reported CPU share is the light half."""
+    A, B = _light_then_heavy(dtype)
+    ref = reference(A, B)
+    with dbd_amd.Context(block_rows=512, dtype=dtype, hybrid=True, cpu_threads=4) as ctx:
+        ctx.set_operands(A, B)
+        info = ctx.plan()
+        assert info["blocks_cpu_pref"] == 20 and info["blocks_any"] == 20, info
+        t = ctx.run("d2h")
+        C = ctx.result()
+    compare(C, ref, dtype, "hybrid d2h")
+    assert t["cpu_blocks"] == 20 and t["cpu_products"] == 10240 * 16 and t["cpu_s"] > 0, t
+    # off by default: the GPU does every block
+    with dbd_amd.Context(block_rows=512, dtype=dtype) as ctx:
+        ctx.set_operands(A, B)
+        t = ctx.run("d2h")
+        assert t["cpu_blocks"] == 0
+        compare(ctx.result(), ref, dtype, "d2h without hybrid")
+
+
+def test_hybrid_io_files_and_cli_report(gpu_ok, tmp_path):
+    A, B = _light_then_heavy(np.float64)
+    ref = reference(A, B)
+    with dbd_amd.Context(block_rows=512, hybrid=True, out_dir=str(tmp_path), chunks_hint=3) as ctx:
+        ctx.set_operands(A, B)
+        t = ctx.run("io")
+    p = np.fromfile(tmp_path / "Ptr.output", dtype=np.int32).astype(np.int64)
+    i = np.fromfile(tmp_path / "Idx.output", dtype=np.int32)
+    v = np.fromfile(tmp_path / "Val.output", dtype=np.float64)
+    compare(dbd_amd.CSR(p, i, v, (A.shape[0], B.shape[1])), ref, np.float64, "hybrid io")
+    assert t["cpu_blocks"] == 20 and t["io_bytes"] == 4 * (A.shape[0] + 1) + 12 * i.size
+    # the CLI's workload distribution lines report the CPU's share
+    for name, M in (("a.mtx", A), ("b.mtx", B)):
+        with open(tmp_path / name, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate real general\n")
+            f.write(f"{M.shape[0] - 1} {M.shape[1] - 1} {M.nnz}\n")
+            rows = np.repeat(np.arange(M.shape[0]), np.diff(M.ptr))
+            f.write("".join(f"{r_} {c_} {float(v_):.17g}\n" for r_, c_, v_ in zip(rows, M.idx, M.val)))
+    binp = os.path.join(ROOT, "dbd_amd", "bin", "SSpMM")
+    r = subprocess.run([binp, "--fp64", "--no-cache", "--hybrid", "--block-rows", "512", str(tmp_path / "a.mtx"),
+                        str(tmp_path / "b.mtx")], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"CPUs\tBlocks:\s+(\d+) \(([0-9.]+) %\)\n\tElements: (\d+)", r.stdout)
+    assert m and int(m.group(1)) == 20 and int(m.group(3)) == 10240 * 16, r.stdout
+    assert float(re.search(r"Compute CPU\t\t([0-9.]+) s", r.stdout).group(1)) > 0
+    i2 = np.fromfile(tmp_path / "Idx.output", dtype=np.int32)
+    assert np.array_equal(i2, i)
+
+
+def test_hybrid_needs_host_operands(gpu_ok):
+    A, B = _light_then_heavy(np.float64)
+    dA, dB = dbd_amd.DeviceCSR.from_host(A), dbd_amd.DeviceCSR.from_host(B)
+    with dbd_amd.Context(block_rows=512, hybrid=True) as ctx:
+        ctx.set_operands(dA, dB)
+        with pytest.raises(dbd_amd.DBDError) as e:
+            ctx.run("d2h")
+        assert e.value.status == 1 and "host operand" in str(e.value)
+        t = ctx.run("compute")  # ignored in compute mode
+        assert t["cpu_blocks"] == 0
