@@ -1,7 +1,9 @@
 """The other named configurations, run through `python bench.py --workload NAME`.
 
     web     web-like 281,903^2 C = A*A, FP64, 4096-row blocks (literal Zipf 2.1 out-degree and
-            the nnz-matched exponent 1.66), all three windows, full-size oracle on the host
+            the nnz-matched exponent 1.66), all three windows, the D2H and files windows again
+            with the hybrid CPU co-execution mode on (CPU share reported), full-size oracle on the
+            host
     rmat22  R-MAT scale 22 (a,b,c,d = .57,.19,.19,.05), C = A*A, FP64, 512-row blocks, 64 GB
             device budget: kernel window over the whole product (streamed in chunks), the whole
             product copied to pinned host memory (mode token d2hfull), D2H and file windows on
@@ -83,6 +85,43 @@ def _windows(A, B, block_rows, dtype, steps, warmup, modes, device_gib=32, host_
     return P, info, out
 
 
+def _hybrid_windows(A, B, block_rows, dtype, steps, io=True, cpu_threads=0):
+    """The D2H and files windows with the hybrid mode on: the blocks the reference's
+    analyze_Block sends to the CPU (average products per row <= 64) run on host threads. Reports
+    the CPU's share of blocks and products and its busy time beside the windows."""
+    import dbd_amd
+    out = {}
+    ctx = dbd_amd.Context(block_rows=block_rows, dtype=dtype, device_gib=32, host_gib=48, hybrid=True,
+                          cpu_threads=cpu_threads)
+    ctx.set_operands(A, B)
+    info = ctx.plan()
+    P = info["products"]
+
+    def share(tm):
+        return {"cpu_blocks": tm["cpu_blocks"], "blocks": info["num_blocks"],
+                "cpu_products_share": round(tm["cpu_products"] / max(P, 1), 4), "cpu_s": round(tm["cpu_s"], 4),
+                "compute_gpu_s": round(tm["compute_gpu_s"], 4)}
+
+    t, tm = _timed(ctx, "d2h", max(1, min(steps, 3)), 1)
+    out["d2h"] = {"ms": round(t * 1e3, 3), "gflops": round(2 * P / t / 1e9, 3), **share(tm)}
+    ctx.close()
+    if io:
+        d = tempfile.mkdtemp(prefix="dbd_hio_", dir="/tmp")
+        c2 = dbd_amd.Context(block_rows=block_rows, dtype=dtype, device_gib=32, host_gib=48, out_dir=d,
+                             writer_threads=8, chunks_hint=16, hybrid=True, cpu_threads=cpu_threads)
+        c2.set_operands(A, B)
+        c2.run("io")
+        for f in ("Ptr.output", "Idx.output", "Val.output"):
+            os.remove(os.path.join(d, f))
+        t0 = time.perf_counter()
+        tm = c2.run("io")
+        t = time.perf_counter() - t0
+        out["io"] = {"ms": round(t * 1e3, 3), "gflops": round(2 * P / t / 1e9, 3), **share(tm)}
+        c2.close()
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def _cpu_windows(A, B, threads, io=True):
     """The oracle as the CPU baseline: compute, + materialise the host CSR (already the
     oracle's output form), + write the same three files to the same kind of sink."""
@@ -113,7 +152,11 @@ def run_web(args):
         A = dbd_amd.gen_web(281903, zipf_s=s, seed=1)
         log(f"web s={s}: nnz {A.nnz}")
         P, info, win = _windows(A, None, 4096, np.float64, args.steps, args.warmup, modes)
-        row = {"zipf_s": s, "nnz_A": A.nnz, "products": P, "nnz_C": info["nnz"], "windows": win}
+        row = {"zipf_s": s, "nnz_A": A.nnz, "products": P, "nnz_C": info["nnz"], "windows": win,
+               "blocks": {"criteria": info["criteria"], "gpu_only": info["blocks_gpu_only"],
+                          "cpu_preferred": info["blocks_cpu_pref"], "any": info["blocks_any"]}}
+        if "d2h" in modes:
+            row["hybrid_windows"] = _hybrid_windows(A, None, 4096, np.float64, args.steps, io="io" in modes)
         if not args.no_cpu:
             th = _threads()
             cpu, _ = _cpu_windows(A, A, th, io="io" in modes)
