@@ -2722,7 +2722,10 @@ static int grid_for(int64_t nrows, int gpb, int grid, const void* f = nullptr, i
 
 constexpr int kLongNT = 256;
 constexpr int kSymLongW = 131072;  // bits per symbolic window (16 KB bitmap)
-constexpr int kNumLongBW = 65536;   // columns per numeric window (8 KB bitmap + 4 KB word prefixes)
+#ifndef DBD_NUM_LONG_BW
+#define DBD_NUM_LONG_BW 65536
+#endif
+constexpr int kNumLongBW = DBD_NUM_LONG_BW;  // columns per numeric window (bitmap + word prefixes)
 constexpr int kNumLongVC64 = 4096;  // FP64 values per window (32 KB; with 8 KB of cursors: 3 workgroups per CU)
 constexpr int kNumLongVC32 = 8192;  // FP32 values per window (32 KB)
 constexpr int kBwdLongW64 = 8192;
