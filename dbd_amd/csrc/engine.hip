@@ -315,10 +315,9 @@ struct dbd_ctx {
   // band index of B's long rows (for the long-row window kernels), built once per operand set
   DevBuf band_hist, band_flag, band_tmp, band_slot, band_row, band;
   bool band_ready = false;
-  // workspace of the hub-row kernels (zero between launches)
-  DevBuf hubpool;
-  HubPool hp;
-  bool hub_dirty = false;
+  // workspace of the hub-row range kernels: banded-entry lists (per A nonzero), per-row counts,
+  // per-(row, range) short-product counts and column counts with their scans, short products
+  DevBuf hub_list, hub_cnt, hub_scnt, hub_rcnt, hub_tmp, hub_scol, hub_sval;
   int64_t band_minlen = 0, band_rows = 0, band_nb1 = 0;
   std::vector<int64_t> h_blk, h_blockP, h_blockC, h_offs_num, h_sym_tot;
   dbd_plan_info info{};
@@ -358,33 +357,47 @@ dbd_status check_err(dbd_ctx* c, const char* phase) {
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  if (e != 0) {
-    c->hub_dirty = true;  // a kernel stopped early: the hub pool may hold values
-    return set_error(DBD_ERR_INTERNAL, "%s: device consistency check failed (flags 0x%x)", phase, e);
-  }
+  if (e != 0) return set_error(DBD_ERR_INTERNAL, "%s: device consistency check failed (flags 0x%x)", phase, e);
   return DBD_OK;
 }
 
-// Hub-row pool: one slice of ncols presence bytes and ncols values per workgroup of the hub
-// kernels' persistent grid, as many slices as fit min(4 GiB, device budget / 8), at most one per
-// compute unit. Zeroed when allocated, and again after a failed launch (the kernels leave it zero).
-dbd_status ensure_hub(dbd_ctx* c) {
-  const int64_t stride = (c->B.ncols + 255) / 256 * 256;
-  const int64_t slice = stride * (1 + (int64_t)c->vsz);
-  if (!c->hubpool.p || c->hp.stride != stride || c->hp.slice_bytes != slice) {
-    const int64_t cap = std::min<int64_t>((int64_t)4 << 30, (c->o.device_budget_mib << 20) / 8);
-    const int64_t n = std::max<int64_t>(1, std::min<int64_t>(c->num_cu, cap / slice));
-    CHK(c->hubpool.ensure((size_t)(n * slice)));
-    c->hp.base = c->hubpool.as<uint8_t>();
-    c->hp.stride = stride;
-    c->hp.slice_bytes = slice;
-    c->hp.slices = (int)n;
-    c->hub_dirty = true;
-  }
-  if (c->hub_dirty) {
-    launch_fill_bytes0(c->hubpool.p, (int64_t)c->hp.slices * c->hp.slice_bytes, c->st);
-    c->hub_dirty = false;
-  }
+// Whether hub rows can take the range kernels: they read B's band index for long B rows and
+// keep one counter per column range in LDS.
+bool hub_ranges_ok(const dbd_ctx* c) { return c->B.band != nullptr && hub_ranges(c->B.ncols) <= kHubMaxRanges; }
+
+// Hub rows rows_num[pos0, pos0 + n) of a chunk in column ranges: banded-entry lists and bucketed
+// short products, a count pass, a scan of the (row, range) counts into output offsets, the
+// numeric pass. One host synchronisation (the number of short products sizes their buckets).
+dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
+                       void* Cval) {
+  hipStream_t s = c->st;
+  const int nrng = hub_ranges(c->B.ncols);
+  const int64_t items = n * nrng;
+  const int32_t* rows = c->rows_num.as<int32_t>() + pos0;
+  CHK(c->hub_list.ensure(4 * (size_t)std::max<int64_t>(c->A.nnz, 1)));
+  CHK(c->hub_cnt.ensure(4 * (size_t)n));
+  CHK(c->hub_scnt.ensure(16 * (size_t)(items + 1)));  // counts, then their scan
+  CHK(c->hub_rcnt.ensure(16 * (size_t)(items + 1)));
+  CHK(c->hub_tmp.ensure(8 * (size_t)scan_tmp_elems(items + 1)));
+  int64_t* scnt = c->hub_scnt.as<int64_t>();
+  int64_t* soff = scnt + items + 1;
+  int64_t* rcnt = c->hub_rcnt.as<int64_t>();
+  int64_t* roff = rcnt + items + 1;
+  launch_hub_prepare(rows, n, c->A, c->B, nrng, scnt, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(), s);
+  launch_exclusive_scan(scnt, soff, items, c->hub_tmp.as<int64_t>(), s);
+  int64_t nshort = 0;
+  HIPCHK(hipMemcpyAsync(&nshort, soff + items, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  CHK(c->hub_scol.ensure(4 * (size_t)std::max<int64_t>(nshort, 1)));
+  CHK(c->hub_sval.ensure(c->vsz * (size_t)std::max<int64_t>(nshort, 1)));
+  launch_hub_scatter(c->o.dtype, rows, n, c->A, c->B, nrng, soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, s);
+  launch_hub_range(c->o.dtype, true, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(),
+                   soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, rcnt, roff, Cptr, cbase, Cidx, Cval,
+                   c->err.as<int>(), s);
+  launch_exclusive_scan(rcnt, roff, items, c->hub_tmp.as<int64_t>(), s);
+  launch_hub_range(c->o.dtype, false, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(),
+                   soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, rcnt, roff, Cptr, cbase, Cidx, Cval,
+                   c->err.as<int>(), s);
   return DBD_OK;
 }
 
@@ -569,10 +582,13 @@ dbd_status numeric_blocks(dbd_ctx* c, int64_t b0, int64_t b1, const int64_t* Cpt
     const int64_t i0 = (int64_t)k * c->nwg + w0, i1 = (int64_t)k * c->nwg + w1;
     const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
     if (n <= 0) continue;
-    if (c->tb.num_k[k].T == 0) CHK(ensure_cursor(c));
-    if (c->tb.num_k[k].T == kHubShape) CHK(ensure_hub(c));
+    if (c->tb.num_k[k].T == kHubShape && hub_ranges_ok(c) && n * hub_ranges(c->B.ncols) < INT32_MAX) {
+      CHK(hub_numeric(c, c->h_offs_num[(size_t)i0], n, Cptr, cbase, Cidx, Cval));
+      continue;
+    }
+    if (c->tb.num_k[k].T == 0 || c->tb.num_k[k].T == kHubShape) CHK(ensure_cursor(c));
     launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
-                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st, distinct, c->hp);
+                   Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st, distinct);
   }
   return DBD_OK;
 }
@@ -617,7 +633,8 @@ int64_t fixed_device_bytes(dbd_ctx* c) {
   s += (int64_t)c->cnt.bytes + c->offs_sym.bytes + c->offs_num.bytes + c->tmp.bytes + c->cursor.bytes;
   s += (int64_t)c->bdesc_s.bytes + c->bdesc_l.bytes;
   s += (int64_t)c->band_hist.bytes + c->band_flag.bytes + c->band_tmp.bytes + c->band_slot.bytes + c->band_row.bytes +
-       c->band.bytes + c->hubpool.bytes;
+       c->band.bytes + c->hub_list.bytes + c->hub_cnt.bytes + c->hub_scnt.bytes + c->hub_rcnt.bytes + c->hub_tmp.bytes +
+       c->hub_scol.bytes + c->hub_sval.bytes;
   return s;
 }
 
@@ -997,7 +1014,8 @@ dbd_status dbd_destroy(dbd_ctx* c) {
   if (c->cst) (void)hipStreamSynchronize(c->cst);
   for (auto* b : {&c->P, &c->Pptr, &c->nnz, &c->Cptr, &c->rows_sym, &c->rows_num, &c->cnt, &c->offs_sym, &c->offs_num,
                   &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng, &c->bdesc_s, &c->bdesc_l, &c->band_hist,
-                  &c->band_flag, &c->band_tmp, &c->band_slot, &c->band_row, &c->band, &c->hubpool})
+                  &c->band_flag, &c->band_tmp, &c->band_slot, &c->band_row, &c->band, &c->hub_list, &c->hub_cnt,
+                  &c->hub_scnt, &c->hub_rcnt, &c->hub_tmp, &c->hub_scol, &c->hub_sval})
     b->release();
   for (int i = 0; i < 3; ++i) {
     c->upA[i].release();
@@ -1218,12 +1236,10 @@ static int persistent_grid(int num_cu, const KernelShape& k) {
 // The whole pipeline (row analysis, bins, symbolic, scan, bins, numeric) on stream s with
 // persistent category kernels that read their row ranges from device memory: no host
 // synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
-// the numeric kernels drain empty ranges and the capacity bit is set in the error word.
+// the numeric kernels drain empty ranges and the capacity bit is set in the error word. Hub rows
+// stay in the long category here (their range kernels size buckets on the host).
 void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu,
-                             int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s,
-                             const HubPool& hub) {
-  // without a pool, or on problems the table keeps off the hub kernels, hub rows stay long
-  const int64_t* hub_rows = hub.slices > 0 && hub_enabled(tb, B.ncols) ? A.ptr : nullptr;
+                             int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s) {
   const int64_t nr = w.nr;
   A.bstart = w.bstart;
   A.blen = w.blen;
@@ -1238,15 +1254,14 @@ void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B,
                     persistent_grid(num_cu, tb.sym_k[k]), A, B, w.nnz, w.cursor, w.err, s);
   launch_exclusive_scan(w.nnz, d_Cptr, nr, w.tmp, s);
   const CatTable& nt = tb.num_ct;
-  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s, hub_rows);
+  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s);
   launch_exclusive_scan(w.cnt, w.offs_num, (int64_t)nt.n * w.nwg, w.tmp, s);
-  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s, hub_rows);
+  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s);
   launch_capacity_guard(d_Cptr + nr, cap, w.offs_num, (int64_t)nt.n * w.nwg + 1, w.err, s);
   for (int k = 1; k < nt.n; ++k) {
-    if (tb.num_k[k].T == kHubShape && !hub_rows) continue;
+    if (tb.num_k[k].T == kHubShape) continue;  // empty: no hub routing in the binning above
     launch_numeric(dtype, tb.num_k[k], w.rows_num, w.offs_num + (int64_t)k * w.nwg, w.offs_num + (int64_t)(k + 1) * w.nwg,
-                   0, persistent_grid(num_cu, tb.num_k[k]), A, B, d_Cptr, 0, d_Cidx, d_Cval, w.cursor, w.err, s, false,
-                   hub);
+                   0, persistent_grid(num_cu, tb.num_k[k]), A, B, d_Cptr, 0, d_Cidx, d_Cval, w.cursor, w.err, s, false);
   }
 }
 
@@ -1269,7 +1284,7 @@ static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int3
   w.err = c->err.as<int>();
   w.nr = c->nr;
   w.nwg = c->nwg;
-  enqueue_pipeline(w, c->tb, c->A, c->B, c->o.dtype, c->num_cu, d_Cptr, d_Cidx, d_Cval, c->batch_cap, s, c->hp);
+  enqueue_pipeline(w, c->tb, c->A, c->B, c->o.dtype, c->num_cu, d_Cptr, d_Cidx, d_Cval, c->batch_cap, s);
   return DBD_OK;
 }
 
@@ -1314,16 +1329,11 @@ dbd_status dbd_batch_launch(dbd_ctx* c, int64_t* d_Cptr, int32_t* d_Cidx, void* 
     HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
     memcpy(c->g_key, key, sizeof(key));
   }
-  if (c->hub_dirty && c->hubpool.p) {  // the graph's hub kernels expect a clear pool
-    launch_fill_bytes0(c->hubpool.p, (int64_t)c->hp.slices * c->hp.slice_bytes, c->st);
-    c->hub_dirty = false;
-  }
   HIPCHK(hipGraphLaunch(c->gexec, c->st));
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->symb = false;
-  if (e != 0) c->hub_dirty = true;
   if (e & kErrCapacityBit)
     return set_error(DBD_ERR_BUDGET, "batch: C needs more than the %lld entries dbd_batch_prepare reported "
                      "(operand structure changed since); nothing was written", (long long)c->batch_cap);

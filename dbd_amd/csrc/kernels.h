@@ -43,18 +43,8 @@ struct KernelShape {
 };
 constexpr int kHubShape = -1;
 // Rows with more A entries than this, among the rows of the long (last by key) category, form the
-// hub category: one workgroup per row streams every B row once into a dense slice of the pool.
+// hub category: their output is computed in column ranges, one workgroup per (row, range).
 constexpr int64_t kHubEntries = 512;
-
-// Workspace of the hub-row kernels: `slices` slices of `slice_bytes`, one per workgroup of their
-// persistent grid; a slice holds `stride` presence bytes (stride >= ncols, a multiple of 256), then
-// stride values. Zero when handed to a kernel; the kernels leave it zero.
-struct HubPool {
-  uint8_t* base = nullptr;
-  int64_t stride = 0;
-  int64_t slice_bytes = 0;
-  int slices = 0;
-};
 
 // ---- generic device utilities ----
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
@@ -116,7 +106,23 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
 void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                     const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                     const int64_t* Cptr, int64_t cbase, int32_t* Cidx, void* Cval, int64_t* cursor, int* err,
-                    hipStream_t s, bool distinct = false, const HubPool& hub = HubPool{});
+                    hipStream_t s, bool distinct = false);
+// Hub rows rows[0, nh) in column ranges of 16384 (hub_ranges(ncols) of them; B's band index
+// required). prepare: per (row, range) count of the short entries' products into scnt[nh *
+// nrng], each row's banded entries (offsets in the row) into hlist at the row's A positions and
+// their number into hcount. scatter: those products, (column, value), into buckets at soff (the
+// exclusive scan of scnt). range(count): distinct columns per (row, range) into rcnt;
+// range(!count): values, emitted at Cptr[row] - cbase + roff[item] - roff[row's first item].
+constexpr int kHubMaxRanges = 4096;
+int hub_ranges(int64_t ncols);
+void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng, int64_t* scnt,
+                        int32_t* hlist, int32_t* hcount, hipStream_t s);
+void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng,
+                        const int64_t* soff, int32_t* scol, void* sval, hipStream_t s);
+void launch_hub_range(int dtype, bool count, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
+                      const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
+                      const void* sval, int64_t* rcnt, const int64_t* roff, const int64_t* Cptr, int64_t cbase,
+                      int32_t* Cidx, void* Cval, int* err, hipStream_t s);
 void launch_backward(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                      const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                      const int64_t* Cptr, const int32_t* Cidx, const void* dC, void* dA, void* dB, int64_t* cursor,

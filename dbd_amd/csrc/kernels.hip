@@ -1181,163 +1181,211 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 // ------------------------------------------------------------------------------------
 // A numeric window kernel visits every A entry of its row in every dense window of 4096 columns
 // it sweeps, so a row's cost grows as nA * ncols / 4096 beside its products: over a few million
-// columns a row of thousands of entries spends more on those visits than on its products. A hub
-// row instead streams each entry's B row once: the products of all its entries are gathered
-// element-parallel over a 1024-thread workgroup (entry ranges flattened by a scan, chunks of NT
-// entries) and accumulated into the workgroup's own presence bytes and dense values in a workspace
-// pool (one slice of `stride` columns per workgroup of a persistent grid), then one sweep over the
-// row's column span emits the columns in order and clears what was set. Its price is a global
-// atomic per product, which is why only wide problems route rows here (the table's hub_min_cols).
-// Each slice is touched by one workgroup only; a device-scope fence orders the accumulation
-// before the sweep. The symbolic pass counts hub rows with the bitmap windows, whose 131072-column
-// windows make the visits cheap.
-template <typename V, int NT>
-__device__ __forceinline__ void hub_accumulate(int t, int64_t a0, int64_t nA, const int64_t* __restrict__ bstart,
-                                               const int32_t* __restrict__ blen, const V* __restrict__ Aval,
-                                               const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
-                                               uint8_t* __restrict__ pres, V* __restrict__ dval, int64_t* hb0,
-                                               V* hva, int64_t* hS, long long* scl) {
-  constexpr int U = 4;
-  for (int64_t e0 = 0; e0 < nA; e0 += NT) {  // uniform: chunks of NT entries
-    long long cnt = 0;
-    if (e0 + t < nA) {
-      const int64_t ka = a0 + e0 + t;
-      cnt = blen[ka];
-      hb0[t] = bstart[ka];
-      hva[t] = Aval[ka];
+// columns a row of thousands of entries spends more on those visits than on its products, and
+// the whole row is one workgroup's work. Hub rows are cut into ranges of kHubRange columns
+// instead, one workgroup per (row, range), each accumulating its range in LDS (a presence bitmap
+// and, for the numeric pass, dense values), so a heavy row spreads over the GPU and no product
+// needs a global atomic:
+//   - entries whose B row is in the band index (long B rows) read their range's element span
+//     from two index entries; `k_hub_prepare` lists them per row (in A's order) and counts the
+//     products of the other (short) entries per range; `k_hub_scatter` writes those products,
+//     (column, a_ik * b_kj), into per-(row, range) buckets;
+//   - `k_hub_range<true>` counts each range's columns, a scan turns the counts into output
+//     offsets, and `k_hub_range<false>` accumulates the range and emits it in column order at
+//     its offset.
+// Within a bucket the products' order follows LDS atomics, so sums of a column may associate
+// differently from run to run (within the usual rounding bound).
+constexpr int kHubRangeShift = 14;
+constexpr int kHubRange = 1 << kHubRangeShift;
+constexpr int kHubRangeBands = kHubRange >> kBandShift;
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_hub_prepare(const int32_t* __restrict__ rows, const int64_t* __restrict__ Aptr,
+                                                    const int32_t* __restrict__ Aidx,
+                                                    const int64_t* __restrict__ bstart,
+                                                    const int32_t* __restrict__ blen, const int32_t* __restrict__ Bidx,
+                                                    const int32_t* __restrict__ hslot, int nrng,
+                                                    int64_t* __restrict__ scnt, int32_t* __restrict__ hlist,
+                                                    int32_t* __restrict__ hcount) {
+  extern __shared__ int lcnt[];  // nrng counters
+  __shared__ int sc[NT / 64];
+  const int t = threadIdx.x;
+  const int64_t j = blockIdx.x;
+  const int row = rows[j];
+  const int64_t a0 = Aptr[row], nA = Aptr[row + 1] - a0;
+  for (int r = t; r < nrng; r += NT) lcnt[r] = 0;
+  __syncthreads();
+  int nH = 0;  // banded entries listed so far (uniform)
+  for (int64_t e0 = 0; e0 < nA; e0 += NT) {  // uniform
+    const int64_t e = e0 + t;
+    int banded = 0;
+    if (e < nA) {
+      const int64_t ka = a0 + e;
+      if (hslot[Aidx[ka]] >= 0) {
+        banded = 1;
+      } else {
+        const int64_t b0 = bstart[ka];
+        const int L = blen[ka];
+        for (int q = 0; q < L; ++q) atomicAdd(&lcnt[Bidx[b0 + q] >> kHubRangeShift], 1);
+      }
     }
-    long long T;  // elements of the chunk (64-bit: a chunk of long B rows can pass 2^31)
-    hS[t] = Group<NT>::exscan(cnt, T, scl);
-    const int nch = (int)min((int64_t)NT, nA - e0);
+    int tot;
+    const int pre = Group<NT>::exscan(banded, tot, sc);
+    if (banded) hlist[a0 + nH + pre] = (int32_t)e;
+    nH += tot;
+  }
+  __syncthreads();
+  for (int r = t; r < nrng; r += NT) scnt[j * nrng + r] = lcnt[r];
+  if (t == 0) hcount[j] = nH;
+}
+
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void k_hub_scatter(const int32_t* __restrict__ rows, const int64_t* __restrict__ Aptr,
+                                                    const int32_t* __restrict__ Aidx,
+                                                    const int64_t* __restrict__ bstart,
+                                                    const int32_t* __restrict__ blen, const V* __restrict__ Aval,
+                                                    const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
+                                                    const int32_t* __restrict__ hslot, int nrng,
+                                                    const int64_t* __restrict__ soff, int32_t* __restrict__ scol,
+                                                    V* __restrict__ sval) {
+  extern __shared__ unsigned long long lcur[];  // nrng write positions
+  const int t = threadIdx.x;
+  const int64_t j = blockIdx.x;
+  const int row = rows[j];
+  const int64_t a0 = Aptr[row], nA = Aptr[row + 1] - a0;
+  for (int r = t; r < nrng; r += NT) lcur[r] = (unsigned long long)soff[j * nrng + r];
+  __syncthreads();
+  for (int64_t e = t; e < nA; e += NT) {
+    const int64_t ka = a0 + e;
+    if (hslot[Aidx[ka]] >= 0) continue;
+    const int64_t b0 = bstart[ka];
+    const int L = blen[ka];
+    const V av = Aval[ka];
+    for (int q = 0; q < L; ++q) {
+      const int col = Bidx[b0 + q];
+      const int64_t pos = (int64_t)atomicAdd(&lcur[col >> kHubRangeShift], 1ull);
+      scol[pos] = col;
+      sval[pos] = av * Bval[b0 + q];
+    }
+  }
+}
+
+// One (row, range) item per workgroup. COUNT: the range's distinct columns into rcnt[item].
+// Otherwise: the range's values, emitted at Cptr[row] - cbase + roff[item] - roff[first item of
+// the row], with roff[item + 1] - roff[item] columns expected.
+template <typename V, int NT, bool COUNT>
+__global__ __launch_bounds__(NT) void k_hub_range(
+    const int32_t* __restrict__ rows, int nrng, const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx,
+    const int64_t* __restrict__ bstart, const V* __restrict__ Aval, const int32_t* __restrict__ Bidx,
+    const V* __restrict__ Bval, const int32_t* __restrict__ hslot, const int32_t* __restrict__ band, int64_t nb1,
+    const int32_t* __restrict__ hlist, const int32_t* __restrict__ hcount, const int64_t* __restrict__ soff,
+    const int32_t* __restrict__ scol, const V* __restrict__ sval, int64_t* __restrict__ rcnt,
+    const int64_t* __restrict__ roff, const int64_t* __restrict__ Cptr, int64_t cbase, int32_t* __restrict__ Cidx,
+    V* __restrict__ Cval, int* __restrict__ err) {
+  constexpr int NWORD = kHubRange / 32;
+  constexpr int NW = NT / 64;
+  constexpr int U = 4;
+  static_assert(NWORD <= NT, "one bitmap word per thread at most");
+  __shared__ uint32_t bm[NWORD];
+  __shared__ V dv[COUNT ? 1 : kHubRange];
+  __shared__ int64_t hb0[NT];
+  __shared__ V hva[COUNT ? 1 : NT];
+  __shared__ int hS[NT];
+  __shared__ int sc[NW];
+  __shared__ long long scl[NW];
+  const int t = threadIdx.x;
+  const int64_t item = blockIdx.x;
+  const int64_t j = item / nrng;
+  const int r = (int)(item - j * nrng);
+  const int row = rows[j];
+  const int64_t a0 = Aptr[row];
+  const int nH = hcount[j];
+  const int64_t c0 = (int64_t)r << kHubRangeShift;
+  const int64_t w0 = c0 >> kBandShift;
+  const int64_t w1 = min(w0 + kHubRangeBands, nb1 - 1);
+  for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
+  if constexpr (!COUNT)
+    for (int q = t; q < kHubRange; q += NT) dv[q] = V(0);
+  __syncthreads();
+  // products of the banded entries in this range, element-parallel over the workgroup
+  for (int e0 = 0; e0 < nH; e0 += NT) {  // uniform
+    int len = 0;
+    if (e0 + t < nH) {
+      const int64_t ka = a0 + hlist[a0 + e0 + t];
+      const int32_t* bd = band + (int64_t)hslot[Aidx[ka]] * nb1;
+      const int s0 = bd[w0];
+      len = bd[w1] - s0;
+      hb0[t] = bstart[ka] + s0;
+      if constexpr (!COUNT) hva[t] = Aval[ka];
+    }
+    int T;
+    hS[t] = Group<NT>::exscan(len, T, sc);
+    const int nch = min(NT, nH - e0);
     __syncthreads();
-    for (int64_t f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+    for (int f0 = 0; f0 < T; f0 += NT * U) {  // uniform
       int col[U];
       V bv[U], va[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int64_t f = f0 + u * NT + t;
-        int j = 0;  // largest j with hS[j] <= f: the chunk entry whose B row holds element f
+        const int f = f0 + u * NT + t;
+        int jj = 0;  // largest jj with hS[jj] <= f
 #pragma unroll
         for (int step = NT / 2; step >= 1; step >>= 1)
-          if (j + step < nch && hS[j + step] <= f) j += step;
-        const int64_t at = f < T ? hb0[j] + (f - hS[j]) : 0;  // lanes past T: unused read
+          if (jj + step < nch && hS[jj + step] <= f) jj += step;
+        const int64_t at = f < T ? hb0[jj] + (f - hS[jj]) : 0;  // lanes past T: unused read of B's first entry
         col[u] = Bidx[at];
-        bv[u] = Bval[at];
-        va[u] = hva[j];
+        if constexpr (!COUNT) {
+          bv[u] = Bval[at];
+          va[u] = hva[jj];
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (f0 + u * NT + t < T) {
-          pres[col[u]] = 1;
-          atomicAdd(&dval[col[u]], va[u] * bv[u]);
+          const int o = (int)(col[u] - c0);
+          atomicOr(&bm[o >> 5], 1u << (o & 31));
+          if constexpr (!COUNT) lds_add(&dv[o], va[u] * bv[u]);
         }
     }
     __syncthreads();  // the chunk tables are rewritten by the next chunk
   }
-}
-
-// Column span [lo, hi] of a row: first and last B column over its entries (B rows are sorted).
-template <int NT>
-__device__ __forceinline__ void hub_span(int t, int64_t a0, int64_t nA, const int64_t* __restrict__ bstart,
-                                         const int32_t* __restrict__ blen, const int32_t* __restrict__ Bidx, int* sc,
-                                         int& lo, int& hi) {
-  int l = INT_MAX, h = -1;
-  for (int64_t e = t; e < nA; e += NT) {
-    const int64_t ka = a0 + e;
-    const int L = blen[ka];
-    if (L > 0) {
-      l = min(l, Bidx[bstart[ka]]);
-      h = max(h, Bidx[bstart[ka] + L - 1]);
-    }
+  // products of the short entries, from the range's bucket
+  const int64_t q0 = soff[item], q1 = soff[item + 1];
+  for (int64_t q = q0 + t; q < q1; q += NT) {
+    const int o = (int)(scol[q] - c0);
+    atomicOr(&bm[o >> 5], 1u << (o & 31));
+    if constexpr (!COUNT) lds_add(&dv[o], sval[q]);
   }
-  lo = Group<NT>::reduce(l, OpMin(), sc);
-  hi = Group<NT>::reduce(h, OpMax(), sc);
-}
-
-template <typename V, int NT>
-__global__ __launch_bounds__(NT) void k_numeric_hub(const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg,
-                                                    const int64_t* __restrict__ pend, const int64_t* __restrict__ Aptr,
-                                                    const int64_t* __restrict__ bstart,
-                                                    const int32_t* __restrict__ blen, const V* __restrict__ Aval,
-                                                    const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
-                                                    const int64_t* __restrict__ Cptr, int64_t cbase,
-                                                    int32_t* __restrict__ Cidx, V* __restrict__ Cval,
-                                                    uint8_t* __restrict__ pool, int64_t stride,
-                                                    int64_t slice_bytes, int* __restrict__ err) {
-  constexpr int NW = NT / 64;
-  __shared__ int64_t hb0[NT];
-  __shared__ V hva[NT];
-  __shared__ int64_t hS[NT];
-  __shared__ int red_cnt[NW];
-  __shared__ int sc[NW];
-  __shared__ long long scl[NW];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  uint8_t* pres = pool + (int64_t)blockIdx.x * slice_bytes;
-  V* dval = reinterpret_cast<V*>(pres + stride);
-  int flags = 0;
-  const int64_t beg = *pbeg, end = *pend;
-  for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
-    const int row = rows[slot];
-    const int64_t a0 = Aptr[row], nA = Aptr[row + 1] - a0;
-    const int64_t c0 = Cptr[row] - cbase, c1 = Cptr[row + 1] - cbase;
-    int lo, hi;
-    hub_span<NT>(t, a0, nA, bstart, blen, Bidx, sc, lo, hi);
-    hub_accumulate<V, NT>(t, a0, nA, bstart, blen, Aval, Bidx, Bval, pres, dval, hb0, hva, hS, scl);
-    __threadfence();
-    __syncthreads();
-    // sweep [lo, hi] in rounds of 16 columns per thread: present columns in order, at positions
-    // from a group scan of the per-thread counts; values read and cleared with the bytes
-    int64_t out = c0;
-    if (hi >= lo) {
-      const int64_t q0 = lo >> 4, q1 = hi >> 4;
-      uint4* pw = reinterpret_cast<uint4*>(pres);
-      for (int64_t qb = q0; qb <= q1; qb += NT) {  // uniform
-        const int64_t q = qb + t;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (q <= q1) v = pw[q];
-        const int cnt = __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
-        const int incl = wave_incl_scan_dpp(cnt);
-        if (lane == 63) red_cnt[wv] = incl;
-        __syncthreads();
-        int base = 0, tot = 0;
-#pragma unroll
-        for (int i = 0; i < NW; ++i) {
-          const int ci = red_cnt[i];
-          base += i < wv ? ci : 0;
-          tot += ci;
-        }
-        if (cnt) {
-          int64_t pos = out + base + incl - cnt;
-          const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            uint32_t bits = w4[k];  // one byte per column, 0 or 1
-            while (bits) {
-              const int b = (__ffs(bits) - 1) >> 3;
-              bits &= ~(0xFFu << (8 * b));
-              const int64_t col = q * 16 + k * 4 + b;
-              if (pos < c1) {
-                Cidx[pos] = (int)col;
-                Cval[pos] = dval[col];
-              } else {
-                flags |= kErrWindow;
-              }
-              dval[col] = V(0);
-              ++pos;
-            }
-          }
-          pw[q] = make_uint4(0, 0, 0, 0);
-        }
-        out += tot;
-        __syncthreads();  // reduction slots free for the next round
+  __syncthreads();
+  const uint32_t w = t < NWORD ? bm[t] : 0u;
+  const int pc = __popc(w);
+  if constexpr (COUNT) {
+    const int tot = Group<NT>::reduce(pc, OpAdd(), sc);
+    if (t == 0) rcnt[item] = tot;
+  } else {
+    int tot;
+    const int pre = Group<NT>::exscan(pc, tot, sc);
+    const int64_t base = roff[item] - roff[j * nrng];
+    const int64_t out = Cptr[row] - cbase + base;
+    const int64_t c1 = Cptr[row + 1] - cbase;
+    int flags = (roff[item + 1] - roff[item] != tot) ? kErrWindow : 0;
+    uint32_t bits = w;
+    int64_t pos = out + pre;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      const int o = 32 * t + b;
+      if (pos < c1) {
+        Cidx[pos] = (int)(c0 + o);
+        Cval[pos] = dv[o];
+      } else {
+        flags |= kErrWindow;
       }
+      ++pos;
     }
-    if (out != c1) flags |= kErrWindow;
-    __threadfence();
-    __syncthreads();  // cleared before the next row of this workgroup
+    flags = wave_reduce_dpp(flags, OpOr());
+    if ((t & 63) == 0 && flags) atomicOr(err, flags);
   }
-  flags = wave_reduce_dpp(flags, OpOr());
-  if (lane == 0 && flags) atomicOr(err, flags);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2747,11 +2795,6 @@ int symbolic_groups_per_wg(const KernelShape& k) {
   return k.NT <= 64 ? 256 / k.NT : 1;
 }
 
-static int hub_grid(int64_t nrows_hint, int grid, const HubPool& hub) {
-  if (grid > 0) return hub.slices;  // persistent launches drain any number of rows
-  return (int)std::max<int64_t>(1, std::min<int64_t>(nrows_hint, hub.slices));
-}
-
 void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                      int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
                      int* err, hipStream_t s) {
@@ -2782,17 +2825,11 @@ template <typename V>
 static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                              int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr,
                              int64_t cbase, int32_t* Cidx, V* Cval, int64_t* cursor, int* err, hipStream_t s,
-                             bool distinct, const HubPool& hub) {
+                             bool distinct) {
   const int gpb = symbolic_groups_per_wg(k);
   const V* Av = (const V*)A.val;
   const V* Bv = (const V*)B.val;
-  if (k.T == kHubShape) {
-    hipLaunchKernelGGL((k_numeric_hub<V, 1024>), dim3(hub_grid(nrows_hint, grid, hub)), dim3(1024), 0, s, rows, pbeg,
-                       pend, A.ptr, A.bstart, A.blen, Av, B.idx, Bv, Cptr, cbase, Cidx, Cval, hub.base, hub.stride,
-                       hub.slice_bytes, err);
-    return;
-  }
-  if (k.T == 0) {
+  if (k.T == 0 || k.T == kHubShape) {  // hub rows reach here only when the range kernels cannot run
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     constexpr int VC = sizeof(V) == 8 ? kNumLongVC64 : kNumLongVC32;
     hipLaunchKernelGGL((k_numeric_long<V, kLongNT, kNumLongBW, VC>), gr, dim3(kLongNT), 0, s, rows, pbeg, pend, A.ptr,
@@ -2826,14 +2863,65 @@ static void launch_numeric_t(const KernelShape& k, const int32_t* rows, const in
 
 void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                     int64_t nrows_hint, int grid, const Mat& A, const Mat& B, const int64_t* Cptr, int64_t cbase,
-                    int32_t* Cidx, void* Cval, int64_t* cursor, int* err, hipStream_t s, bool distinct,
-                    const HubPool& hub) {
+                    int32_t* Cidx, void* Cval, int64_t* cursor, int* err, hipStream_t s, bool distinct) {
   if (dtype == 1)
     launch_numeric_t<double>(k, rows, pbeg, pend, nrows_hint, grid, A, B, Cptr, cbase, Cidx, (double*)Cval, cursor,
-                             err, s, distinct, hub);
+                             err, s, distinct);
   else
     launch_numeric_t<float>(k, rows, pbeg, pend, nrows_hint, grid, A, B, Cptr, cbase, Cidx, (float*)Cval, cursor,
-                            err, s, distinct, hub);
+                            err, s, distinct);
+}
+
+int hub_ranges(int64_t ncols) { return (int)((ncols + kHubRange - 1) >> kHubRangeShift); }
+
+void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng, int64_t* scnt,
+                        int32_t* hlist, int32_t* hcount, hipStream_t s) {
+  if (nh <= 0) return;
+  hipLaunchKernelGGL(k_hub_prepare<256>, dim3((unsigned)nh), dim3(256), (size_t)nrng * sizeof(int), s, rows, A.ptr,
+                     A.idx, A.bstart, A.blen, B.idx, B.hslot, nrng, scnt, hlist, hcount);
+}
+
+void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng,
+                        const int64_t* soff, int32_t* scol, void* sval, hipStream_t s) {
+  if (nh <= 0) return;
+  const size_t lds = (size_t)nrng * sizeof(unsigned long long);
+  if (dtype == 1)
+    hipLaunchKernelGGL((k_hub_scatter<double, 256>), dim3((unsigned)nh), dim3(256), lds, s, rows, A.ptr, A.idx,
+                       A.bstart, A.blen, (const double*)A.val, B.idx, (const double*)B.val, B.hslot, nrng, soff, scol,
+                       (double*)sval);
+  else
+    hipLaunchKernelGGL((k_hub_scatter<float, 256>), dim3((unsigned)nh), dim3(256), lds, s, rows, A.ptr, A.idx,
+                       A.bstart, A.blen, (const float*)A.val, B.idx, (const float*)B.val, B.hslot, nrng, soff, scol,
+                       (float*)sval);
+}
+
+template <typename V>
+static void launch_hub_range_t(bool count, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
+                               const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
+                               const V* sval, int64_t* rcnt, const int64_t* roff, const int64_t* Cptr, int64_t cbase,
+                               int32_t* Cidx, V* Cval, int* err, hipStream_t s) {
+  const dim3 gr((unsigned)(nh * nrng));
+  if (count)
+    hipLaunchKernelGGL((k_hub_range<V, 1024, true>), gr, dim3(1024), 0, s, rows, nrng, A.ptr, A.idx, A.bstart,
+                       (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist, hcount, soff, scol,
+                       sval, rcnt, roff, Cptr, cbase, Cidx, Cval, err);
+  else
+    hipLaunchKernelGGL((k_hub_range<V, 1024, false>), gr, dim3(1024), 0, s, rows, nrng, A.ptr, A.idx, A.bstart,
+                       (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist, hcount, soff, scol,
+                       sval, rcnt, roff, Cptr, cbase, Cidx, Cval, err);
+}
+
+void launch_hub_range(int dtype, bool count, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
+                      const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
+                      const void* sval, int64_t* rcnt, const int64_t* roff, const int64_t* Cptr, int64_t cbase,
+                      int32_t* Cidx, void* Cval, int* err, hipStream_t s) {
+  if (nh <= 0) return;
+  if (dtype == 1)
+    launch_hub_range_t<double>(count, rows, nh, nrng, A, B, hlist, hcount, soff, scol, (const double*)sval, rcnt, roff,
+                               Cptr, cbase, Cidx, (double*)Cval, err, s);
+  else
+    launch_hub_range_t<float>(count, rows, nh, nrng, A, B, hlist, hcount, soff, scol, (const float*)sval, rcnt, roff,
+                              Cptr, cbase, Cidx, (float*)Cval, err, s);
 }
 
 template <typename V>

@@ -22,7 +22,7 @@
 
 namespace dbd {
 
-// Problems of at least this many columns route hub rows to the hub kernels by default.
+// Problems of at least this many columns route hub rows to the hub range kernels by default.
 constexpr int64_t kHubAutoCols = (int64_t)1 << 21;
 
 // A category table: thresholds and kernel shapes of the symbolic and numeric categories, and the
@@ -62,9 +62,8 @@ struct Work {
 // persistent category kernels that read their row ranges from device memory: no host
 // synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
 // the numeric kernels drain empty ranges and the capacity bit is set in the error word.
-// Hub rows take the hub kernels with `hub`'s pool when the table enables them for B's width, and
-// stay in the long category when the pool is empty.
+// Hub rows stay in the long category here.
 void enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu, int64_t* d_Cptr,
-                      int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s, const HubPool& hub = HubPool{});
+                      int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s);
 
 }  // namespace dbd
