@@ -44,7 +44,7 @@ struct KernelShape {
 constexpr int kHubShape = -1;
 // Rows with more A entries than this, among the rows of the long (last by key) category, form the
 // hub category: their output is computed in column ranges, one workgroup per (row, range).
-constexpr int64_t kHubEntries = 512;
+constexpr int64_t kHubEntries = 2048;
 
 // ---- generic device utilities ----
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
