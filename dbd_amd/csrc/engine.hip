@@ -318,6 +318,10 @@ struct dbd_ctx {
   // workspace of the hub-row range kernels: banded-entry lists (per A nonzero), per-row counts,
   // per-(row, range) short-product counts and column counts with their scans, short products
   DevBuf hub_list, hub_cnt, hub_scnt, hub_rcnt, hub_tmp, hub_scol, hub_sval;
+  // hub slots of the symbolic long rows of more than kHubEntries entries (per row, -1: none) and
+  // their column counts per hub range, written by the symbolic long kernel
+  DevBuf hub_slot, hub_flag, hub_rng;
+  int64_t hub_n = 0;
   int64_t band_minlen = 0, band_rows = 0, band_nb1 = 0;
   std::vector<int64_t> h_blk, h_blockP, h_blockC, h_offs_num, h_sym_tot;
   dbd_plan_info info{};
@@ -365,8 +369,39 @@ dbd_status check_err(dbd_ctx* c, const char* phase) {
 // keep one counter per column range in LDS.
 bool hub_ranges_ok(const dbd_ctx* c) { return c->B.band != nullptr && hub_ranges(c->B.ncols) <= kHubMaxRanges; }
 
+// Hub slots (after the symbolic binning and the band index): the symbolic long rows of more than
+// kHubEntries A entries, when the table sends hub rows to the range kernels for this width. The
+// symbolic long kernel then records their column counts per range, and the numeric binning puts
+// the ones that stay long into the hub category.
+dbd_status assign_hub_slots(dbd_ctx* c) {
+  c->hub_n = 0;
+  const CatTable& ct = c->tb.sym_ct;
+  const int64_t p0 = c->h_sym_tot[(size_t)ct.n - 1], nlong = c->h_sym_tot[(size_t)ct.n] - p0;
+  if (nlong <= 0 || !hub_enabled(c->tb, c->B.ncols) || !hub_ranges_ok(c)) return DBD_OK;
+  hipStream_t s = c->st;
+  const int nrng = hub_ranges(c->B.ncols);
+  const int32_t* rows = c->rows_sym.as<int32_t>() + p0;
+  CHK(c->hub_slot.ensure(4 * (size_t)c->nr));
+  CHK(c->hub_flag.ensure(16 * (size_t)(nlong + 1)));
+  CHK(c->hub_tmp.ensure(8 * (size_t)scan_tmp_elems(nlong + 1)));
+  int64_t* flag = c->hub_flag.as<int64_t>();
+  int64_t* pos = flag + nlong + 1;
+  launch_fill_i32(c->hub_slot.as<int32_t>(), c->nr, -1, s);
+  launch_hub_flags(rows, nlong, c->A.ptr, flag, s);
+  launch_exclusive_scan(flag, pos, nlong, c->hub_tmp.as<int64_t>(), s);
+  int64_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, pos + nlong, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (n <= 0 || n * (int64_t)nrng >= INT32_MAX) return DBD_OK;  // none, or too many to count per range
+  launch_hub_slots(rows, nlong, flag, pos, c->hub_slot.as<int32_t>(), s);
+  CHK(c->hub_rng.ensure(4 * (size_t)(n * nrng)));
+  launch_fill_i32(c->hub_rng.as<int32_t>(), n * nrng, 0, s);
+  c->hub_n = n;
+  return DBD_OK;
+}
+
 // Hub rows rows_num[pos0, pos0 + n) of a chunk in column ranges: banded-entry lists and bucketed
-// short products, a count pass, a scan of the (row, range) counts into output offsets, the
+// short products, the symbolic pass's (row, range) column counts scanned into output offsets, the
 // numeric pass. One host synchronisation (the number of short products sizes their buckets).
 dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
                        void* Cval) {
@@ -391,9 +426,7 @@ dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr,
   CHK(c->hub_scol.ensure(4 * (size_t)std::max<int64_t>(nshort, 1)));
   CHK(c->hub_sval.ensure(c->vsz * (size_t)std::max<int64_t>(nshort, 1)));
   launch_hub_scatter(c->o.dtype, rows, n, c->A, c->B, nrng, soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, s);
-  launch_hub_range(c->o.dtype, true, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(),
-                   soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, rcnt, roff, Cptr, cbase, Cidx, Cval,
-                   c->err.as<int>(), s);
+  launch_hub_gather_counts(rows, n, nrng, c->hub_slot.as<int32_t>(), c->hub_rng.as<int32_t>(), rcnt, s);
   launch_exclusive_scan(rcnt, roff, items, c->hub_tmp.as<int64_t>(), s);
   launch_hub_range(c->o.dtype, false, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(),
                    soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, rcnt, roff, Cptr, cbase, Cidx, Cval,
@@ -520,6 +553,7 @@ dbd_status plan(dbd_ctx* c) {
   // long rows (the last symbolic category) read B's band index
   if (c->h_sym_tot[(size_t)ct.n] - c->h_sym_tot[(size_t)ct.n - 1] > 0 && !c->band_ready) CHK(build_band(c));
   attach_band(c);
+  CHK(assign_hub_slots(c));
   c->planned = true;
   c->symb = false;
   return DBD_OK;
@@ -538,9 +572,11 @@ dbd_status symbolic(dbd_ctx* c) {
     const int64_t n = c->h_sym_tot[(size_t)k + 1] - c->h_sym_tot[(size_t)k];
     if (n <= 0) continue;
     if (c->tb.sym_k[k].T == 0) CHK(ensure_cursor(c));
+    const bool hubs = c->hub_n > 0;
     launch_symbolic(c->tb.sym_k[k], c->rows_sym.as<int32_t>(), offs + (int64_t)k * c->nwg,
                     offs + (int64_t)(k + 1) * c->nwg, n, 0, c->A, c->B, c->nnz.as<int64_t>(),
-                    c->cursor.as<int64_t>(), c->err.as<int>(), s);
+                    c->cursor.as<int64_t>(), c->err.as<int>(), s, hubs ? c->hub_slot.as<int32_t>() : nullptr,
+                    hubs ? c->hub_rng.as<int32_t>() : nullptr, hubs ? hub_ranges(c->B.ncols) : 0);
   }
   launch_exclusive_scan(c->nnz.as<int64_t>(), c->Cptr.as<int64_t>(), c->nr, c->tmp.as<int64_t>(), s);
   const int BR = c->o.block_rows;
@@ -550,7 +586,7 @@ dbd_status symbolic(dbd_ctx* c) {
   // numeric bins by exact nnz_i
   const CatTable& nt = c->tb.num_ct;
   const int64_t n_off = (int64_t)nt.n * c->nwg;
-  const int64_t* hub_rows = hub_enabled(c->tb, c->B.ncols) ? c->A.ptr : nullptr;
+  const int32_t* hub_rows = c->hub_n > 0 ? c->hub_slot.as<int32_t>() : nullptr;
   launch_bin_count(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->cnt.as<int64_t>(), s, hub_rows);
   launch_exclusive_scan(c->cnt.as<int64_t>(), c->offs_num.as<int64_t>(), n_off, c->tmp.as<int64_t>(), s);
   launch_bin_scatter(c->nnz.as<int64_t>(), c->nr, INT64_MAX, nt, c->offs_num.as<int64_t>(), c->rows_num.as<int32_t>(),
@@ -582,7 +618,7 @@ dbd_status numeric_blocks(dbd_ctx* c, int64_t b0, int64_t b1, const int64_t* Cpt
     const int64_t i0 = (int64_t)k * c->nwg + w0, i1 = (int64_t)k * c->nwg + w1;
     const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
     if (n <= 0) continue;
-    if (c->tb.num_k[k].T == kHubShape && hub_ranges_ok(c) && n * hub_ranges(c->B.ncols) < INT32_MAX) {
+    if (c->tb.num_k[k].T == kHubShape) {  // rows with hub slots (hub_n > 0): the range kernels apply
       CHK(hub_numeric(c, c->h_offs_num[(size_t)i0], n, Cptr, cbase, Cidx, Cval));
       continue;
     }
@@ -634,7 +670,7 @@ int64_t fixed_device_bytes(dbd_ctx* c) {
   s += (int64_t)c->bdesc_s.bytes + c->bdesc_l.bytes;
   s += (int64_t)c->band_hist.bytes + c->band_flag.bytes + c->band_tmp.bytes + c->band_slot.bytes + c->band_row.bytes +
        c->band.bytes + c->hub_list.bytes + c->hub_cnt.bytes + c->hub_scnt.bytes + c->hub_rcnt.bytes + c->hub_tmp.bytes +
-       c->hub_scol.bytes + c->hub_sval.bytes;
+       c->hub_scol.bytes + c->hub_sval.bytes + c->hub_slot.bytes + c->hub_flag.bytes + c->hub_rng.bytes;
   return s;
 }
 
@@ -1015,7 +1051,8 @@ dbd_status dbd_destroy(dbd_ctx* c) {
   for (auto* b : {&c->P, &c->Pptr, &c->nnz, &c->Cptr, &c->rows_sym, &c->rows_num, &c->cnt, &c->offs_sym, &c->offs_num,
                   &c->tmp, &c->cursor, &c->blk, &c->err, &c->g_rng, &c->bdesc_s, &c->bdesc_l, &c->band_hist,
                   &c->band_flag, &c->band_tmp, &c->band_slot, &c->band_row, &c->band, &c->hub_list, &c->hub_cnt,
-                  &c->hub_scnt, &c->hub_rcnt, &c->hub_tmp, &c->hub_scol, &c->hub_sval})
+                  &c->hub_scnt, &c->hub_rcnt, &c->hub_tmp, &c->hub_scol, &c->hub_sval, &c->hub_slot, &c->hub_flag,
+                  &c->hub_rng})
     b->release();
   for (int i = 0; i < 3; ++i) {
     c->upA[i].release();

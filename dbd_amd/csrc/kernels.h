@@ -44,7 +44,13 @@ struct KernelShape {
 constexpr int kHubShape = -1;
 // Rows with more A entries than this, among the rows of the long (last by key) category, form the
 // hub category: their output is computed in column ranges, one workgroup per (row, range).
-constexpr int64_t kHubEntries = 2048;
+#ifndef DBD_HUB_ENTRIES
+#define DBD_HUB_ENTRIES 2048
+#endif
+constexpr int64_t kHubEntries = DBD_HUB_ENTRIES;
+// Column ranges of the hub-row kernels (16384 columns); the symbolic long windows hold whole ranges.
+constexpr int kHubRangeShift = 14;
+constexpr int kHubRange = 1 << kHubRangeShift;
 
 // ---- generic device utilities ----
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
@@ -87,19 +93,23 @@ void launch_row_products(const Mat& A, const Mat& B, int64_t* P, int64_t* nnz, i
 // Stable counting sort of rows [0, n) by category of min(key[i], cap).
 // counts/offs have ncat * nwg (+1) int64 where nwg = bin_tiles(n).
 int64_t bin_tiles(int64_t n);
-// With Aptr, rows of the category before the last (the long one by key) that have more than
-// kHubEntries A entries go to the last category (the hub category).
+// With hub_slot, rows of the category before the last (the long one by key) that have a hub slot
+// (>= 0) go to the last category (the hub category).
 void launch_bin_count(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, int64_t* counts,
-                      hipStream_t s, const int64_t* Aptr = nullptr);
+                      hipStream_t s, const int32_t* hub_slot = nullptr);
 void launch_bin_scatter(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, const int64_t* offs,
-                        int32_t* rows, hipStream_t s, const int64_t* Aptr = nullptr);
+                        int32_t* rows, hipStream_t s, const int32_t* hub_slot = nullptr);
 
 // ---- symbolic / numeric / backward per category ----
 // grid == 0 -> exact grid from nrows_hint; else a persistent grid of that many workgroups.
 int symbolic_groups_per_wg(const KernelShape& k);
+// With hub_slot, the long-row kernel gives rows of slot hs >= 0 windows aligned to whole hub ranges
+// and writes their columns per range into rng_cnt[hs * nrng + range] (ranges with no column are
+// left as they are: zero them first).
 void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                      int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
-                     int* err, hipStream_t s);
+                     int* err, hipStream_t s, const int32_t* hub_slot = nullptr, int32_t* rng_cnt = nullptr,
+                     int nrng = 0);
 // distinct: most products of the problem are distinct columns of their row (nnzC close to the
 // product count); the one-wave shapes then take rows whose products are all distinct through a
 // register path instead of the hash table.
@@ -115,6 +125,14 @@ void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const 
 // range(!count): values, emitted at Cptr[row] - cbase + roff[item] - roff[row's first item].
 constexpr int kHubMaxRanges = 4096;
 int hub_ranges(int64_t ncols);
+// Hub slots: flag[j] = 1 for rows[j] with more than kHubEntries A entries; with pos the exclusive
+// scan of flag, hub_slot[rows[j]] = pos[j] for the flagged rows (hub_slot filled with -1 first).
+void launch_hub_flags(const int32_t* rows, int64_t n, const int64_t* Aptr, int64_t* flag, hipStream_t s);
+void launch_hub_slots(const int32_t* rows, int64_t n, const int64_t* flag, const int64_t* pos, int32_t* hub_slot,
+                      hipStream_t s);
+// rcnt[j * nrng + r] = rng_cnt[hub_slot[rows[j]] * nrng + r] (the symbolic pass's range counts).
+void launch_hub_gather_counts(const int32_t* rows, int64_t n, int nrng, const int32_t* hub_slot,
+                              const int32_t* rng_cnt, int64_t* rcnt, hipStream_t s);
 void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng, int64_t* scnt,
                         int32_t* hlist, int32_t* hcount, hipStream_t s);
 void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng,

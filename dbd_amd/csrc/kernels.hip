@@ -895,19 +895,20 @@ __device__ __forceinline__ int category_of(int64_t k, const CatTable& ct) {
   return ct.n - 1;
 }
 
-// Category of row i: by key. Tables end with the long category (limit INT64_MAX, so no key goes
-// past it) and the hub category; with Aptr, a long row with more than kHubEntries A entries is a
-// hub row. Without Aptr the hub category stays empty.
-__device__ __forceinline__ int row_category(int64_t k, const CatTable& ct, const int64_t* __restrict__ Aptr,
+// Category of row i: by key. Numeric tables end with the long category (limit INT64_MAX, so no
+// key goes past it) and the hub category; with hub_slot, a long row that has a hub slot (a
+// symbolic long row of more than kHubEntries A entries, whose range counts the symbolic pass
+// wrote) is a hub row. Without hub_slot the hub category stays empty.
+__device__ __forceinline__ int row_category(int64_t k, const CatTable& ct, const int32_t* __restrict__ hub_slot,
                                             int64_t i) {
   const int c = category_of(k, ct);
-  if (Aptr && c == ct.n - 2 && Aptr[i + 1] - Aptr[i] > kHubEntries) return ct.n - 1;
+  if (hub_slot && c == ct.n - 2 && hub_slot[i] >= 0) return ct.n - 1;
   return c;
 }
 
 __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restrict__ key, int64_t n, int64_t cap,
                                                         CatTable ct, int64_t* __restrict__ counts, int64_t nwg,
-                                                        const int64_t* __restrict__ Aptr) {
+                                                        const int32_t* __restrict__ hub_slot) {
   __shared__ int hist[kMaxCats];
   if (threadIdx.x < kMaxCats) hist[threadIdx.x] = 0;
   __syncthreads();
@@ -915,7 +916,7 @@ __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restric
   if (i < n) {
     int64_t k = key[i];
     if (k > cap) k = cap;
-    atomicAdd(&hist[row_category(k, ct, Aptr, i)], 1);
+    atomicAdd(&hist[row_category(k, ct, hub_slot, i)], 1);
   }
   __syncthreads();
   if (threadIdx.x < ct.n) counts[(int64_t)threadIdx.x * nwg + blockIdx.x] = hist[threadIdx.x];
@@ -923,14 +924,15 @@ __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restric
 
 __global__ __launch_bounds__(kBinTile) void k_bin_scatter(const int64_t* __restrict__ key, int64_t n, int64_t cap,
                                                           CatTable ct, const int64_t* __restrict__ offs, int64_t nwg,
-                                                          int32_t* __restrict__ rows, const int64_t* __restrict__ Aptr) {
+                                                          int32_t* __restrict__ rows,
+                                                          const int32_t* __restrict__ hub_slot) {
   __shared__ int wc[kBinTile / 64][kMaxCats];
   const int64_t i = (int64_t)blockIdx.x * kBinTile + threadIdx.x;
   int c = -1;
   if (i < n) {
     int64_t k = key[i];
     if (k > cap) k = cap;
-    c = row_category(k, ct, Aptr, i);
+    c = row_category(k, ct, hub_slot, i);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -948,17 +950,17 @@ __global__ __launch_bounds__(kBinTile) void k_bin_scatter(const int64_t* __restr
 }
 
 void launch_bin_count(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, int64_t* counts,
-                      hipStream_t s, const int64_t* Aptr) {
+                      hipStream_t s, const int32_t* hub_slot) {
   const int64_t nwg = bin_tiles(n);
   if (nwg == 0) return;
-  hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, counts, nwg, Aptr);
+  hipLaunchKernelGGL(k_bin_count, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, counts, nwg, hub_slot);
 }
 void launch_bin_scatter(const int64_t* key, int64_t n, int64_t cap, const CatTable& ct, const int64_t* offs,
-                        int32_t* rows, hipStream_t s, const int64_t* Aptr) {
+                        int32_t* rows, hipStream_t s, const int32_t* hub_slot) {
   const int64_t nwg = bin_tiles(n);
   if (nwg == 0) return;
   hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)nwg), dim3(kBinTile), 0, s, key, n, cap, ct, offs, nwg, rows,
-                     Aptr);
+                     hub_slot);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1076,23 +1078,28 @@ __device__ __forceinline__ int long_lanes(int64_t nA) {
 // comes from the row analysis' descriptors. An entry with more elements in the window than one
 // step of G continues UL steps at a time, their loads issued together.
 template <int NT, int W, int G>
-__global__ __launch_bounds__(NT) void k_symbolic_long(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_symbolic_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
     const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
     const int32_t* __restrict__ Bidx, int64_t* __restrict__ nnz, int64_t* __restrict__ cursor,
-    int* __restrict__ err) {
+    int* __restrict__ err, const int32_t* __restrict__ hub_slot, int32_t* __restrict__ rng_cnt, int nrng) {
   constexpr int NWORD = W / 32;
   constexpr int WPT = NWORD / NT;
   constexpr int UL = 8;
-  static_assert(NWORD % NT == 0, "shape");
+  constexpr int RPW = W >> kHubRangeShift;          // hub ranges per window
+  constexpr int RWORD = NWORD / RPW;                // bitmap words per hub range
+  static_assert(NWORD % NT == 0 && W % kHubRange == 0, "shape");
   __shared__ uint32_t bm[NWORD];
   __shared__ int sc[NT / 64 > 0 ? NT / 64 : 1];
   __shared__ long long scl[NT / 64 > 0 ? NT / 64 : 1];
+  __shared__ int rc[RPW];
   const int t = threadIdx.x, lg = t % G;
   const int64_t beg = *pbeg, end = *pend;
   for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
     const int row = rows[slot];
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
+    // rows with a hub slot take windows aligned to W and record their columns per hub range
+    const int hs = hub_slot ? hub_slot[row] : -1;
 #pragma unroll
     for (int q = 0; q < WPT; ++q) bm[t + q * NT] = 0u;
     int lo_l = INT_MAX;
@@ -1110,6 +1117,7 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
     long long count = 0;
     long long windows = 0;
     while (lo != INT_MAX) {
+      if (hs >= 0) lo &= ~(W - 1);  // uniform
       const int64_t hi = min((int64_t)lo + W, (int64_t)INT_MAX);  // INT_MAX: past the B row
       for (int64_t ka = a0 + t / G; ka < a1; ka += NT / G) {
         int64_t p = cursor[ka];
@@ -1151,14 +1159,22 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
         }
         if (lg == 0) cursor[ka] = p;
       }
+      if (hs >= 0 && t < RPW) rc[t] = 0;
       __syncthreads();
       long long pc = 0;
 #pragma unroll
       for (int q = 0; q < WPT; ++q) {
-        pc += __popc(bm[t + q * NT]);
+        const int wq = __popc(bm[t + q * NT]);
+        pc += wq;
+        if (hs >= 0 && wq) atomicAdd(&rc[(t + q * NT) / RWORD], wq);
         bm[t + q * NT] = 0u;
       }
       count += pc;
+      if (hs >= 0) {  // uniform
+        __syncthreads();
+        const int r0 = lo >> kHubRangeShift;
+        if (t < RPW && r0 + t < nrng) rng_cnt[(int64_t)hs * nrng + r0 + t] = rc[t];
+      }
       int nl = INT_MAX;
       for (int64_t ka = a0 + t; ka < a1; ka += NT) {
         const int64_t p = cursor[ka];
@@ -1195,8 +1211,6 @@ __global__ __launch_bounds__(NT) void k_symbolic_long(
 //     its offset.
 // Within a bucket the products' order follows LDS atomics, so sums of a column may associate
 // differently from run to run (within the usual rounding bound).
-constexpr int kHubRangeShift = 14;
-constexpr int kHubRange = 1 << kHubRangeShift;
 constexpr int kHubRangeBands = kHubRange >> kBandShift;
 
 template <int NT>
@@ -2797,12 +2811,12 @@ int symbolic_groups_per_wg(const KernelShape& k) {
 
 void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* pbeg, const int64_t* pend,
                      int64_t nrows_hint, int grid, const Mat& A, const Mat& B, int64_t* nnz, int64_t* cursor,
-                     int* err, hipStream_t s) {
+                     int* err, hipStream_t s, const int32_t* hub_slot, int32_t* rng_cnt, int nrng) {
   const int gpb = symbolic_groups_per_wg(k);
   if (k.T == 0 || k.T == kHubShape) {  // the symbolic pass counts hub rows with the long-row windows
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     hipLaunchKernelGGL((k_symbolic_long<1024, kSymLongW, kLongG>), gr, dim3(1024), 0, s, rows, pbeg, pend, A.ptr,
-                       A.bstart, A.blen, B.idx, nnz, cursor, err);
+                       A.bstart, A.blen, B.idx, nnz, cursor, err, hub_slot, rng_cnt, nrng);
     return;
   }
 #define X(T_, NT_, G_)                                                                                         \
@@ -2873,6 +2887,47 @@ void launch_numeric(int dtype, const KernelShape& k, const int32_t* rows, const 
 }
 
 int hub_ranges(int64_t ncols) { return (int)((ncols + kHubRange - 1) >> kHubRangeShift); }
+
+__global__ void k_hub_flags(const int32_t* __restrict__ rows, int64_t n, const int64_t* __restrict__ Aptr,
+                            int64_t* __restrict__ flag) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int r = rows[j];
+    flag[j] = Aptr[r + 1] - Aptr[r] > kHubEntries ? 1 : 0;
+  }
+}
+
+__global__ void k_hub_slots(const int32_t* __restrict__ rows, int64_t n, const int64_t* __restrict__ flag,
+                            const int64_t* __restrict__ pos, int32_t* __restrict__ hub_slot) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    if (flag[j]) hub_slot[rows[j]] = (int32_t)pos[j];
+}
+
+__global__ void k_hub_gather_counts(const int32_t* __restrict__ rows, int64_t n, int nrng,
+                                    const int32_t* __restrict__ hub_slot, const int32_t* __restrict__ rng_cnt,
+                                    int64_t* __restrict__ rcnt) {
+  const int64_t items = n * nrng;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < items; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = q / nrng;
+    const int r = (int)(q - j * nrng);
+    rcnt[q] = rng_cnt[(int64_t)hub_slot[rows[j]] * nrng + r];
+  }
+}
+
+void launch_hub_flags(const int32_t* rows, int64_t n, const int64_t* Aptr, int64_t* flag, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hub_flags, dim3(fill_grid(n)), dim3(256), 0, s, rows, n, Aptr, flag);
+}
+void launch_hub_slots(const int32_t* rows, int64_t n, const int64_t* flag, const int64_t* pos, int32_t* hub_slot,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hub_slots, dim3(fill_grid(n)), dim3(256), 0, s, rows, n, flag, pos, hub_slot);
+}
+void launch_hub_gather_counts(const int32_t* rows, int64_t n, int nrng, const int32_t* hub_slot,
+                              const int32_t* rng_cnt, int64_t* rcnt, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hub_gather_counts, dim3(fill_grid(n * nrng)), dim3(256), 0, s, rows, n, nrng, hub_slot, rng_cnt,
+                     rcnt);
+}
 
 void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng, int64_t* scnt,
                         int32_t* hlist, int32_t* hcount, hipStream_t s) {
