@@ -45,7 +45,7 @@ constexpr int kHubShape = -1;
 // Rows with more A entries than this, among the rows of the long (last by key) category, form the
 // hub category: their output is computed in column ranges, one workgroup per (row, range).
 #ifndef DBD_HUB_ENTRIES
-#define DBD_HUB_ENTRIES 2048
+#define DBD_HUB_ENTRIES 1024
 #endif
 constexpr int64_t kHubEntries = DBD_HUB_ENTRIES;
 // Column ranges of the hub-row kernels (16384 columns); the symbolic long windows hold whole ranges.

@@ -199,7 +199,7 @@ typedef struct dbd_table_desc {
   double load;          /* hash-table load limit: a table of T slots takes rows of <= load*T keys */
   int32_t top;          /* slots of the largest numeric LDS table (4096 or 8192; symbolic 2x);
                            rows beyond it take the long-row windows */
-  int64_t hub_min_cols; /* long rows of more than 2048 A entries take the numeric hub range
+  int64_t hub_min_cols; /* long rows of more than 1024 A entries take the numeric hub range
                            kernels when B has at least this many columns (2^21: by width; 0:
                            always; INT64_MAX: never) */
 } dbd_table_desc;
