@@ -428,9 +428,8 @@ dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr,
   launch_hub_scatter(c->o.dtype, rows, n, c->A, c->B, nrng, soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, s);
   launch_hub_gather_counts(rows, n, nrng, c->hub_slot.as<int32_t>(), c->hub_rng.as<int32_t>(), rcnt, s);
   launch_exclusive_scan(rcnt, roff, items, c->hub_tmp.as<int64_t>(), s);
-  launch_hub_range(c->o.dtype, false, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(),
-                   soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, rcnt, roff, Cptr, cbase, Cidx, Cval,
-                   c->err.as<int>(), s);
+  launch_hub_range(c->o.dtype, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(), soff,
+                   c->hub_scol.as<int32_t>(), c->hub_sval.p, roff, Cptr, cbase, Cidx, Cval, c->err.as<int>(), s);
   return DBD_OK;
 }
 

@@ -137,10 +137,10 @@ void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat
                         int32_t* hlist, int32_t* hcount, hipStream_t s);
 void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng,
                         const int64_t* soff, int32_t* scol, void* sval, hipStream_t s);
-void launch_hub_range(int dtype, bool count, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
+void launch_hub_range(int dtype, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
                       const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
-                      const void* sval, int64_t* rcnt, const int64_t* roff, const int64_t* Cptr, int64_t cbase,
-                      int32_t* Cidx, void* Cval, int* err, hipStream_t s);
+                      const void* sval, const int64_t* roff, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
+                      void* Cval, int* err, hipStream_t s);
 void launch_backward(int dtype, const KernelShape& k, const int32_t* rows, const int64_t* pbeg,
                      const int64_t* pend, int64_t nrows_hint, int grid, const Mat& A, const Mat& B,
                      const int64_t* Cptr, const int32_t* Cidx, const void* dC, void* dA, void* dB, int64_t* cursor,

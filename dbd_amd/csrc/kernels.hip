@@ -1206,9 +1206,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
 //     from two index entries; `k_hub_prepare` lists them per row (in A's order) and counts the
 //     products of the other (short) entries per range; `k_hub_scatter` writes those products,
 //     (column, a_ik * b_kj), into per-(row, range) buckets;
-//   - `k_hub_range<true>` counts each range's columns, a scan turns the counts into output
-//     offsets, and `k_hub_range<false>` accumulates the range and emits it in column order at
-//     its offset.
+//   - the symbolic long kernel counts each range's columns (hub rows' windows are aligned to
+//     whole ranges), a scan turns the counts into output offsets, and `k_hub_range` accumulates
+//     the range and emits it in column order at its offset.
 // Within a bucket the products' order follows LDS atomics, so sums of a column may associate
 // differently from run to run (within the usual rounding bound).
 constexpr int kHubRangeBands = kHubRange >> kBandShift;
@@ -1284,16 +1284,15 @@ __global__ __launch_bounds__(NT) void k_hub_scatter(const int32_t* __restrict__ 
   }
 }
 
-// One (row, range) item per workgroup. COUNT: the range's distinct columns into rcnt[item].
-// Otherwise: the range's values, emitted at Cptr[row] - cbase + roff[item] - roff[first item of
-// the row], with roff[item + 1] - roff[item] columns expected.
-template <typename V, int NT, bool COUNT>
+// One (row, range) item per workgroup: the range's values, emitted at Cptr[row] - cbase +
+// roff[item] - roff[first item of the row], with roff[item + 1] - roff[item] columns expected.
+template <typename V, int NT>
 __global__ __launch_bounds__(NT) void k_hub_range(
     const int32_t* __restrict__ rows, int nrng, const int64_t* __restrict__ Aptr, const int32_t* __restrict__ Aidx,
     const int64_t* __restrict__ bstart, const V* __restrict__ Aval, const int32_t* __restrict__ Bidx,
     const V* __restrict__ Bval, const int32_t* __restrict__ hslot, const int32_t* __restrict__ band, int64_t nb1,
     const int32_t* __restrict__ hlist, const int32_t* __restrict__ hcount, const int64_t* __restrict__ soff,
-    const int32_t* __restrict__ scol, const V* __restrict__ sval, int64_t* __restrict__ rcnt,
+    const int32_t* __restrict__ scol, const V* __restrict__ sval,
     const int64_t* __restrict__ roff, const int64_t* __restrict__ Cptr, int64_t cbase, int32_t* __restrict__ Cidx,
     V* __restrict__ Cval, int* __restrict__ err) {
   constexpr int NWORD = kHubRange / 32;
@@ -1301,12 +1300,11 @@ __global__ __launch_bounds__(NT) void k_hub_range(
   constexpr int U = 4;
   static_assert(NWORD <= NT, "one bitmap word per thread at most");
   __shared__ uint32_t bm[NWORD];
-  __shared__ V dv[COUNT ? 1 : kHubRange];
+  __shared__ V dv[kHubRange];
   __shared__ int64_t hb0[NT];
-  __shared__ V hva[COUNT ? 1 : NT];
+  __shared__ V hva[NT];
   __shared__ int hS[NT];
   __shared__ int sc[NW];
-  __shared__ long long scl[NW];
   const int t = threadIdx.x;
   const int64_t item = blockIdx.x;
   const int64_t j = item / nrng;
@@ -1318,8 +1316,7 @@ __global__ __launch_bounds__(NT) void k_hub_range(
   const int64_t w0 = c0 >> kBandShift;
   const int64_t w1 = min(w0 + kHubRangeBands, nb1 - 1);
   for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
-  if constexpr (!COUNT)
-    for (int q = t; q < kHubRange; q += NT) dv[q] = V(0);
+  for (int q = t; q < kHubRange; q += NT) dv[q] = V(0);
   __syncthreads();
   // products of the banded entries in this range, element-parallel over the workgroup
   for (int e0 = 0; e0 < nH; e0 += NT) {  // uniform
@@ -1330,7 +1327,7 @@ __global__ __launch_bounds__(NT) void k_hub_range(
       const int s0 = bd[w0];
       len = bd[w1] - s0;
       hb0[t] = bstart[ka] + s0;
-      if constexpr (!COUNT) hva[t] = Aval[ka];
+      hva[t] = Aval[ka];
     }
     int T;
     hS[t] = Group<NT>::exscan(len, T, sc);
@@ -1348,17 +1345,15 @@ __global__ __launch_bounds__(NT) void k_hub_range(
           if (jj + step < nch && hS[jj + step] <= f) jj += step;
         const int64_t at = f < T ? hb0[jj] + (f - hS[jj]) : 0;  // lanes past T: unused read of B's first entry
         col[u] = Bidx[at];
-        if constexpr (!COUNT) {
-          bv[u] = Bval[at];
-          va[u] = hva[jj];
-        }
+        bv[u] = Bval[at];
+        va[u] = hva[jj];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (f0 + u * NT + t < T) {
           const int o = (int)(col[u] - c0);
           atomicOr(&bm[o >> 5], 1u << (o & 31));
-          if constexpr (!COUNT) lds_add(&dv[o], va[u] * bv[u]);
+          lds_add(&dv[o], va[u] * bv[u]);
         }
     }
     __syncthreads();  // the chunk tables are rewritten by the next chunk
@@ -1368,38 +1363,33 @@ __global__ __launch_bounds__(NT) void k_hub_range(
   for (int64_t q = q0 + t; q < q1; q += NT) {
     const int o = (int)(scol[q] - c0);
     atomicOr(&bm[o >> 5], 1u << (o & 31));
-    if constexpr (!COUNT) lds_add(&dv[o], sval[q]);
+    lds_add(&dv[o], sval[q]);
   }
   __syncthreads();
   const uint32_t w = t < NWORD ? bm[t] : 0u;
   const int pc = __popc(w);
-  if constexpr (COUNT) {
-    const int tot = Group<NT>::reduce(pc, OpAdd(), sc);
-    if (t == 0) rcnt[item] = tot;
-  } else {
-    int tot;
-    const int pre = Group<NT>::exscan(pc, tot, sc);
-    const int64_t base = roff[item] - roff[j * nrng];
-    const int64_t out = Cptr[row] - cbase + base;
-    const int64_t c1 = Cptr[row + 1] - cbase;
-    int flags = (roff[item + 1] - roff[item] != tot) ? kErrWindow : 0;
-    uint32_t bits = w;
-    int64_t pos = out + pre;
-    while (bits) {
-      const int b = __ffs(bits) - 1;
-      bits &= bits - 1u;
-      const int o = 32 * t + b;
-      if (pos < c1) {
-        Cidx[pos] = (int)(c0 + o);
-        Cval[pos] = dv[o];
-      } else {
-        flags |= kErrWindow;
-      }
-      ++pos;
+  int tot;
+  const int pre = Group<NT>::exscan(pc, tot, sc);
+  const int64_t base = roff[item] - roff[j * nrng];
+  const int64_t out = Cptr[row] - cbase + base;
+  const int64_t c1 = Cptr[row + 1] - cbase;
+  int flags = (roff[item + 1] - roff[item] != tot) ? kErrWindow : 0;
+  uint32_t bits = w;
+  int64_t pos = out + pre;
+  while (bits) {
+    const int b = __ffs(bits) - 1;
+    bits &= bits - 1u;
+    const int o = 32 * t + b;
+    if (pos < c1) {
+      Cidx[pos] = (int)(c0 + o);
+      Cval[pos] = dv[o];
+    } else {
+      flags |= kErrWindow;
     }
-    flags = wave_reduce_dpp(flags, OpOr());
-    if ((t & 63) == 0 && flags) atomicOr(err, flags);
+    ++pos;
   }
+  flags = wave_reduce_dpp(flags, OpOr());
+  if ((t & 63) == 0 && flags) atomicOr(err, flags);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2951,32 +2941,26 @@ void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A
 }
 
 template <typename V>
-static void launch_hub_range_t(bool count, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
+static void launch_hub_range_t(const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
                                const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
-                               const V* sval, int64_t* rcnt, const int64_t* roff, const int64_t* Cptr, int64_t cbase,
-                               int32_t* Cidx, V* Cval, int* err, hipStream_t s) {
-  const dim3 gr((unsigned)(nh * nrng));
-  if (count)
-    hipLaunchKernelGGL((k_hub_range<V, 1024, true>), gr, dim3(1024), 0, s, rows, nrng, A.ptr, A.idx, A.bstart,
-                       (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist, hcount, soff, scol,
-                       sval, rcnt, roff, Cptr, cbase, Cidx, Cval, err);
-  else
-    hipLaunchKernelGGL((k_hub_range<V, 1024, false>), gr, dim3(1024), 0, s, rows, nrng, A.ptr, A.idx, A.bstart,
-                       (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist, hcount, soff, scol,
-                       sval, rcnt, roff, Cptr, cbase, Cidx, Cval, err);
+                               const V* sval, const int64_t* roff, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
+                               V* Cval, int* err, hipStream_t s) {
+  hipLaunchKernelGGL((k_hub_range<V, 1024>), dim3((unsigned)(nh * nrng)), dim3(1024), 0, s, rows, nrng, A.ptr, A.idx,
+                     A.bstart, (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist, hcount, soff,
+                     scol, sval, roff, Cptr, cbase, Cidx, Cval, err);
 }
 
-void launch_hub_range(int dtype, bool count, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
+void launch_hub_range(int dtype, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
                       const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
-                      const void* sval, int64_t* rcnt, const int64_t* roff, const int64_t* Cptr, int64_t cbase,
-                      int32_t* Cidx, void* Cval, int* err, hipStream_t s) {
+                      const void* sval, const int64_t* roff, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
+                      void* Cval, int* err, hipStream_t s) {
   if (nh <= 0) return;
   if (dtype == 1)
-    launch_hub_range_t<double>(count, rows, nh, nrng, A, B, hlist, hcount, soff, scol, (const double*)sval, rcnt, roff,
-                               Cptr, cbase, Cidx, (double*)Cval, err, s);
+    launch_hub_range_t<double>(rows, nh, nrng, A, B, hlist, hcount, soff, scol, (const double*)sval, roff, Cptr, cbase,
+                               Cidx, (double*)Cval, err, s);
   else
-    launch_hub_range_t<float>(count, rows, nh, nrng, A, B, hlist, hcount, soff, scol, (const float*)sval, rcnt, roff,
-                              Cptr, cbase, Cidx, (float*)Cval, err, s);
+    launch_hub_range_t<float>(rows, nh, nrng, A, B, hlist, hcount, soff, scol, (const float*)sval, roff, Cptr, cbase,
+                              Cidx, (float*)Cval, err, s);
 }
 
 template <typename V>
