@@ -1816,6 +1816,7 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 // LDS (sharing the word-prefix space) up to kOldStates entries, else in the workspace `state`
 // (16 bytes per A nonzero). Both paths leave the value array and the bitmap clear.
 constexpr int kNumLongStates = 512;
+constexpr int kSpanCap = 16;  // span-walk windows: elements per window of an entry without a band
 
 
 template <typename V, int NT, int BW, int VC>
@@ -1843,7 +1844,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
   LongState* ostl = reinterpret_cast<LongState*>(lds + DVB + NWORD * 4);
   __shared__ int red[NW], red_cnt[NW];
   __shared__ long long redl[NW];
-  __shared__ long long s_hi2;
+  __shared__ long long s_hi1, s_hi2;
   __shared__ int s_nh, s_nl;
   static_assert(2 * NT * (8 + (int)sizeof(V) + 4) <= NWORD * 2 + kNumLongStates * 16,
                 "banded windows: range tables exceed the LDS they share");
@@ -1873,25 +1874,114 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     uint64_t submask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << sgbase;
     if (nA <= NSG) {
       // ================= two walks per window of up to BW columns =================
+      // With B's band index the walks are element-parallel over the workgroup: every active
+      // entry's element span in the window (banded B rows: up to the band boundary at or past the
+      // window end; other B rows: at most kSpanCap elements, the window ending before the first
+      // element left out) is flattened by a scan and each thread takes U elements at a time, so
+      // an entry with many elements in the window costs no chain of dependent steps. The entry
+      // state (B row start and length, elements consumed, next column) is in lane e of wave 0.
+      // Without the index each entry's subgroup walks it from a cursor in LDS.
+      const bool spans = band != nullptr;  // uniform
       LongState* st = stl;
+      int* hS = reinterpret_cast<int*>(lds + DVB + NWORD * 4 + NWORD * 2);  // span starts (64)
+      int* np = hS + 64;                                                     // consumed in walk 2 (64)
+      int64_t* hb0 = reinterpret_cast<int64_t*>(np + 64);                   // span start positions
+      V* hva = reinterpret_cast<V*>(hb0 + 64);                               // A values
+      int64_t eb0 = 0;
+      int eL = 0, ep = 0, enc = INT_MAX;
+      V eva = V(0);
+      const int32_t* ebd = nullptr;
       int lmin = INT_MAX;
       long long psum = 0;
-      for (int64_t e = t; e < nA; e += NT) {
-        LongState x;
-        x.p = bstart[a0 + e];
-        x.rem = blen[a0 + e];
-        x.nc = x.rem > 0 ? Bidx[x.p] : INT_MAX;
-        st[e] = x;
-        lmin = min(lmin, x.nc);
-        psum += x.rem;
+      if (spans) {
+        if (t < nA) {  // nA <= 64: wave 0
+          const int64_t ka = a0 + t;
+          eb0 = bstart[ka];
+          eL = blen[ka];
+          eva = Aval[ka];
+          const int sl = hslot[Aidx[ka]];
+          ebd = sl >= 0 ? band + (int64_t)sl * nb1 : nullptr;
+          enc = eL > 0 ? Bidx[eb0] : INT_MAX;
+          lmin = enc;
+          psum = eL;
+        }
+      } else {
+        for (int64_t e = t; e < nA; e += NT) {
+          LongState x;
+          x.p = bstart[a0 + e];
+          x.rem = blen[a0 + e];
+          x.nc = x.rem > 0 ? Bidx[x.p] : INT_MAX;
+          st[e] = x;
+          lmin = min(lmin, x.nc);
+          psum += x.rem;
+        }
       }
+      // flattened element f of the window belongs to entry j, the last with hS[j] <= f (f < T)
+      auto span_owner = [&](int f) {
+        int j = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+          if (j + step < nA && hS[j + step] <= f) j += step;
+        return j;
+      };
       // a window always takes the element at its first column: at most P_i windows per row
       const long long prow = Group<NT>::reduce(psum, OpAdd(), redl);
       int lo = block_min(lmin);  // also orders the state writes before the walks
       int64_t width = BW;
       long long windows = 0;
       while (lo != INT_MAX) {
-        const int64_t hi1 = min((int64_t)lo + width, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
+        int64_t hi1 = min((int64_t)lo + width, (int64_t)INT_MAX);  // INT_MAX: exhausted entry
+        int T = 0;
+        if (spans) {
+          // ---- the active entries' element spans; the window ends before any element a
+          // capped span leaves out ----
+          if (wv == 0) {
+            int len = 0, clip = INT_MAX;
+            if (lane < nA && (int64_t)enc < hi1) {
+              int q;
+              if (ebd != nullptr) {
+                q = ebd[min((hi1 + (1 << kBandShift) - 1) >> kBandShift, nb1 - 1)];
+              } else {
+                q = min(eL, ep + kSpanCap);
+                if (q < eL) clip = Bidx[eb0 + q];
+              }
+              len = q - ep;
+            }
+            const int cm = wave_reduce_dpp(clip, OpMin());
+            const int incl = wave_incl_scan_dpp(len);
+            if (lane < nA) {
+              hS[lane] = incl - len;
+              hb0[lane] = eb0 + ep;
+              hva[lane] = eva;
+              np[lane] = 0;
+            }
+            if (lane == 63) {
+              s_nh = incl;
+              s_hi1 = min(hi1, (int64_t)cm);
+            }
+          }
+          __syncthreads();
+          T = s_nh;
+          hi1 = s_hi1;
+          // ---- walk 1: presence bits of [lo, hi1), element-parallel ----
+          for (int f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+            int c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int f = f0 + u * NT + t;
+              const int j = span_owner(f);
+              const int64_t at = f < T ? hb0[j] + (f - hS[j]) : 0;  // lanes past T: unused read
+              c[u] = Bidx[at];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (f0 + u * NT + t < T && (int64_t)c[u] < hi1) {
+                const int o = c[u] - lo;
+                atomicOr(&bm[o >> 5], 1u << (o & 31));
+              }
+            }
+          }
+        } else {
         // ---- walk 1: presence bits of [lo, hi1), per-entry counts ----
         for (int64_t e = sg; e < nA; e += NSG) {  // subgroup-uniform
           LongState x = st[e];
@@ -1927,6 +2017,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
               st[e] = x;
             }
           }
+        }
         }
         __syncthreads();
         // ---- word prefixes, and the cut when more than VC columns are present ----
@@ -1974,8 +2065,49 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         __syncthreads();
         const int64_t hi2 = s_hi2;
         const int nw = min(tot, VC);
-        // ---- walk 2: values of [lo, hi2) at their ranks; cursors advance to hi2 ----
         lmin = INT_MAX;
+        if (spans) {
+          // ---- walk 2: values of [lo, hi2) at their ranks, element-parallel; each entry's
+          // elements below hi2 are a prefix of its span, whose length the last such lane of
+          // every wave records ----
+          for (int f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+            int c[U], jj[U];
+            V b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int f = f0 + u * NT + t;
+              jj[u] = span_owner(f);
+              const int64_t at = f < T ? hb0[jj[u]] + (f - hS[jj[u]]) : 0;
+              c[u] = Bidx[at];
+              b[u] = Bval[at];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int f = f0 + u * NT + t;
+              const int j = jj[u];
+              const bool in = f < T && (int64_t)c[u] < hi2;
+              if (in) {
+                const int o = c[u] - lo;
+                const int w = o >> 5;
+                const int r = (int)pre[w] + __popc(bm[w] & ((1u << (o & 31)) - 1u));
+                lds_add(&dv[r], hva[j] * b[u]);
+              }
+              const int jn = __shfl_down(j, 1);
+              const int inn = __shfl_down((int)in, 1);
+              if (in && (lane == 63 || !inn || jn != j)) atomicMax(&np[j], f - hS[j] + 1);
+            }
+          }
+          __syncthreads();  // np complete
+          if (t < nA) {
+            const int took = np[t];
+            if (took > 0) {
+              ep += took;
+              enc = ep < eL ? Bidx[eb0 + ep] : INT_MAX;
+            }
+            lmin = enc;
+          }
+        } else {
+        // ---- walk 2: values of [lo, hi2) at their ranks; cursors advance to hi2 ----
         for (int64_t e = sg; e < nA; e += NSG) {  // subgroup-uniform
           LongState x = st[e];
           if (x.nc < 0) {
@@ -2017,6 +2149,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             if (lg == 0) st[e] = x;
           }
           lmin = min(lmin, x.nc);
+        }
         }
         __syncthreads();
         // ---- emit: values as they lie, columns from the bitmap; both left clear ----
