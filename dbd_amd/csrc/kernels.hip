@@ -1817,6 +1817,7 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 // (16 bytes per A nonzero). Both paths leave the value array and the bitmap clear.
 constexpr int kNumLongStates = 512;
 constexpr int kSpanCap = 16;  // span-walk windows: elements per window of an entry without a band
+constexpr int kSpanMax = 256;  // span-walk windows: A entries per row at most (one per thread)
 
 
 template <typename V, int NT, int BW, int VC>
@@ -1848,6 +1849,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ int s_nh, s_nl;
   static_assert(2 * NT * (8 + (int)sizeof(V) + 4) <= NWORD * 2 + kNumLongStates * 16,
                 "banded windows: range tables exceed the LDS they share");
+  static_assert(kSpanMax <= NT && kSpanMax % 64 == 0 && kSpanMax * (16 + (int)sizeof(V)) <= kNumLongStates * 16,
+                "span walks: one entry per thread, tables within the state LDS");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
   for (int q = t; q < VC; q += NT) dv[q] = V(0);
@@ -1872,21 +1875,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     int G = long_lanes<NT>(nA);
     int NSG = NT / G, sg = t / G, lg = t & (G - 1), sgbase = lane & ~(G - 1);
     uint64_t submask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << sgbase;
-    if (nA <= NSG) {
+    const bool spans = band != nullptr && nA <= kSpanMax;  // uniform
+    if (spans || nA <= NSG) {
       // ================= two walks per window of up to BW columns =================
       // With B's band index the walks are element-parallel over the workgroup: every active
       // entry's element span in the window (banded B rows: up to the band boundary at or past the
       // window end; other B rows: at most kSpanCap elements, the window ending before the first
       // element left out) is flattened by a scan and each thread takes U elements at a time, so
       // an entry with many elements in the window costs no chain of dependent steps. The entry
-      // state (B row start and length, elements consumed, next column) is in lane e of wave 0.
-      // Without the index each entry's subgroup walks it from a cursor in LDS.
-      const bool spans = band != nullptr;  // uniform
+      // state (B row start and length, elements consumed, next column) is in the registers of
+      // thread e (rows of up to kSpanMax entries). Without the index (rows of up to NT / G
+      // entries) each entry's subgroup walks it from a cursor in LDS.
       LongState* st = stl;
-      int* hS = reinterpret_cast<int*>(lds + DVB + NWORD * 4 + NWORD * 2);  // span starts (64)
-      int* np = hS + 64;                                                     // consumed in walk 2 (64)
-      int64_t* hb0 = reinterpret_cast<int64_t*>(np + 64);                   // span start positions
-      V* hva = reinterpret_cast<V*>(hb0 + 64);                               // A values
+      int* hS = reinterpret_cast<int*>(lds + DVB + NWORD * 4 + NWORD * 2);  // span starts
+      int* np = hS + kSpanMax;                                               // consumed in walk 2
+      int64_t* hb0 = reinterpret_cast<int64_t*>(np + kSpanMax);             // span start positions
+      V* hva = reinterpret_cast<V*>(hb0 + kSpanMax);                         // A values
       int64_t eb0 = 0;
       int eL = 0, ep = 0, enc = INT_MAX;
       V eva = V(0);
@@ -1894,7 +1898,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
       int lmin = INT_MAX;
       long long psum = 0;
       if (spans) {
-        if (t < nA) {  // nA <= 64: wave 0
+        if (t < nA) {
           const int64_t ka = a0 + t;
           eb0 = bstart[ka];
           eL = blen[ka];
@@ -1920,7 +1924,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
       auto span_owner = [&](int f) {
         int j = 0;
 #pragma unroll
-        for (int step = 32; step >= 1; step >>= 1)
+        for (int step = kSpanMax / 2; step >= 1; step >>= 1)
           if (j + step < nA && hS[j + step] <= f) j += step;
         return j;
       };
@@ -1935,30 +1939,41 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (spans) {
           // ---- the active entries' element spans; the window ends before any element a
           // capped span leaves out ----
-          if (wv == 0) {
-            int len = 0, clip = INT_MAX;
-            if (lane < nA && (int64_t)enc < hi1) {
-              int q;
-              if (ebd != nullptr) {
-                q = ebd[min((hi1 + (1 << kBandShift) - 1) >> kBandShift, nb1 - 1)];
-              } else {
-                q = min(eL, ep + kSpanCap);
-                if (q < eL) clip = Bidx[eb0 + q];
+          int len = 0, clip = INT_MAX;
+          if (t < nA && (int64_t)enc < hi1) {
+            int q;
+            if (ebd != nullptr) {
+              q = ebd[min((hi1 + (1 << kBandShift) - 1) >> kBandShift, nb1 - 1)];
+            } else {
+              q = min(eL, ep + kSpanCap);
+              if (q < eL) clip = Bidx[eb0 + q];
+            }
+            len = q - ep;
+          }
+          if (nA <= 64) {  // uniform: wave 0 holds every entry
+            if (wv == 0) {
+              const int cm = wave_reduce_dpp(clip, OpMin());
+              const int incl = wave_incl_scan_dpp(len);
+              if (lane < nA) hS[lane] = incl - len;
+              if (lane == 63) {
+                s_nh = incl;
+                s_hi1 = min(hi1, (int64_t)cm);
               }
-              len = q - ep;
             }
-            const int cm = wave_reduce_dpp(clip, OpMin());
-            const int incl = wave_incl_scan_dpp(len);
-            if (lane < nA) {
-              hS[lane] = incl - len;
-              hb0[lane] = eb0 + ep;
-              hva[lane] = eva;
-              np[lane] = 0;
-            }
-            if (lane == 63) {
-              s_nh = incl;
+          } else {
+            int tot;
+            const int pre0 = Group<NT>::exscan(len, tot, red_cnt);
+            const int cm = Group<NT>::reduce(clip, OpMin(), red);
+            if (t < nA) hS[t] = pre0;
+            if (t == 0) {
+              s_nh = tot;
               s_hi1 = min(hi1, (int64_t)cm);
             }
+          }
+          if (t < nA) {
+            hb0[t] = eb0 + ep;
+            hva[t] = eva;
+            np[t] = 0;
           }
           __syncthreads();
           T = s_nh;
