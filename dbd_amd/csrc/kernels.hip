@@ -1073,27 +1073,52 @@ __device__ __forceinline__ int long_lanes(int64_t nA) {
 }
 
 
-// Rows too long for an LDS table: sweep the row's columns in windows of W bits; one cursor
-// per A entry (cursor[] is indexed like A's nonzeros) walks its sorted B row, whose extent
-// comes from the row analysis' descriptors. An entry with more elements in the window than one
-// step of G continues UL steps at a time, their loads issued together.
+// Span walks (long-row windows, symbolic and numeric): elements per window of an A entry whose B
+// row has no band index entry; the window ends before the first element such a span leaves out.
+constexpr int kSpanCap = 16;
+// The symbolic windows are 32 times wider than the numeric dense windows: a cap of 64 elements
+// (no fewer than the band index's smallest banded length on large problems) keeps them whole.
+constexpr int kSymSpanCap = 64;
+constexpr int kSymSpanEntries = 64;  // symbolic span walks: rows of up to one wave of entries
+
+// Rows too long for an LDS table: sweep the row's columns in windows of W bits.
+// With B's band index, rows of up to kSymSpanEntries entries (no hub slot) walk
+// element-parallel: every active entry's element span in the window (banded B rows: up to the
+// band boundary at or past the window end; other B rows: at most kSymSpanCap elements, the
+// window then ending before the first element left out) is flattened by a scan in wave 0, each
+// thread of the workgroup takes U elements at a time, and the entry state (B row start and
+// length, elements consumed, next column, band slot) lives in LDS. (A workgroup-wide version for
+// rows of up to NT entries measured slower than the cursor walks below: its scans cost more
+// barriers per window than the walks' chains.)
+// Otherwise one cursor per A entry (cursor[] is indexed like A's nonzeros) walks its sorted B
+// row, whose extent comes from the row analysis' descriptors; an entry with more elements in the
+// window than one step of G continues UL steps at a time, their loads issued together.
 template <int NT, int W, int G>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_symbolic_long(
     const int32_t* __restrict__ rows, const int64_t* __restrict__ pbeg, const int64_t* __restrict__ pend,
     const int64_t* __restrict__ Aptr, const int64_t* __restrict__ bstart, const int32_t* __restrict__ blen,
     const int32_t* __restrict__ Bidx, int64_t* __restrict__ nnz, int64_t* __restrict__ cursor,
-    int* __restrict__ err, const int32_t* __restrict__ hub_slot, int32_t* __restrict__ rng_cnt, int nrng) {
+    int* __restrict__ err, const int32_t* __restrict__ hub_slot, int32_t* __restrict__ rng_cnt, int nrng,
+    const int32_t* __restrict__ Aidx, const int32_t* __restrict__ hslot, const int32_t* __restrict__ band,
+    int64_t nb1) {
   constexpr int NWORD = W / 32;
   constexpr int WPT = NWORD / NT;
   constexpr int UL = 8;
+  constexpr int U = 4;
   constexpr int RPW = W >> kHubRangeShift;          // hub ranges per window
   constexpr int RWORD = NWORD / RPW;                // bitmap words per hub range
-  static_assert(NWORD % NT == 0 && W % kHubRange == 0, "shape");
+  static_assert(NWORD % NT == 0 && W % kHubRange == 0 && NT % 64 == 0, "shape");
   __shared__ uint32_t bm[NWORD];
   __shared__ int sc[NT / 64 > 0 ? NT / 64 : 1];
   __shared__ long long scl[NT / 64 > 0 ? NT / 64 : 1];
   __shared__ int rc[RPW];
-  const int t = threadIdx.x, lg = t % G;
+  // span-walk entry state and window tables (entry e in slot e)
+  __shared__ int64_t eB0[kSymSpanEntries];
+  __shared__ int eL[kSymSpanEntries], eP[kSymSpanEntries], eNC[kSymSpanEntries], eSL[kSymSpanEntries];
+  __shared__ int sS[kSymSpanEntries], sN[kSymSpanEntries];
+  __shared__ int s_T, s_lo;
+  __shared__ long long s_hi;
+  const int t = threadIdx.x, lg = t % G, lane = t & 63;
   const int64_t beg = *pbeg, end = *pend;
   for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
     const int row = rows[slot];
@@ -1102,6 +1127,124 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int hs = hub_slot ? hub_slot[row] : -1;
 #pragma unroll
     for (int q = 0; q < WPT; ++q) bm[t + q * NT] = 0u;
+    const int nA = (int)min(a1 - a0, (int64_t)kSymSpanEntries + 1);
+    if (band != nullptr && hs < 0 && nA <= kSymSpanEntries) {  // uniform
+      // ================= span walks (entries in slots 0 .. nA - 1, wave 0) =================
+      long long prow_l = 0;
+      if (t < nA) {
+        const int64_t ka = a0 + t;
+        const int64_t b0 = bstart[ka];
+        const int L = blen[ka];
+        eB0[t] = b0;
+        eL[t] = L;
+        eP[t] = 0;
+        eSL[t] = hslot[Aidx[ka]];
+        eNC[t] = L > 0 ? Bidx[b0] : INT_MAX;
+        prow_l = L;
+      }
+      if (t < 64) {
+        const int m = wave_reduce_dpp(t < nA ? eNC[t] : INT_MAX, OpMin());
+        if (t == 0) s_lo = m;
+      }
+      const long long prow = Group<NT>::reduce(prow_l, OpAdd(), scl);  // also orders the writes
+      int lo = s_lo;
+      long long count = 0;
+      long long windows = 0;
+      while (lo != INT_MAX) {
+        // ---- the active entries' element spans; the window ends before any element a capped
+        // span leaves out ----
+        if (t < 64) {
+          const int64_t hi0 = min((int64_t)lo + W, (int64_t)INT_MAX);
+          int len = 0, clip = INT_MAX;
+          if (t < nA && (int64_t)eNC[t] < hi0) {
+            const int p = eP[t], L = eL[t], sl = eSL[t];
+            int q;
+            if (sl >= 0) {
+              q = band[(int64_t)sl * nb1 + min((hi0 + (1 << kBandShift) - 1) >> kBandShift, nb1 - 1)];
+            } else {
+              q = min(L, p + kSymSpanCap);
+              if (q < L) clip = Bidx[eB0[t] + q];
+            }
+            len = q - p;
+          }
+          const int cm = wave_reduce_dpp(clip, OpMin());
+          const int incl = wave_incl_scan_dpp(len);
+          if (t < nA) {
+            sS[t] = incl - len;
+            sN[t] = 0;
+          }
+          if (t == 63) {
+            s_T = incl;
+            s_hi = min(hi0, (int64_t)cm);
+          }
+        }
+        __syncthreads();
+        const int T = s_T;
+        const int64_t hi = s_hi;
+        // ---- presence bits, element-parallel ----
+        for (int f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+          int c[U], jj[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int f = f0 + u * NT + t;
+            int j = 0;  // the last entry with sS[j] <= f
+#pragma unroll
+            for (int step = kSymSpanEntries / 2; step >= 1; step >>= 1)
+              if (j + step < nA && sS[j + step] <= f) j += step;
+            jj[u] = j;
+            const int64_t at = f < T ? eB0[j] + eP[j] + (f - sS[j]) : 0;  // lanes past T: unused read
+            c[u] = Bidx[at];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int f = f0 + u * NT + t;
+            const int j = jj[u];
+            const bool in = f < T && (int64_t)c[u] < hi;
+            if (in) {
+              const int o = c[u] - lo;
+              atomicOr(&bm[o >> 5], 1u << (o & 31));
+            }
+            // an entry's elements below hi are a prefix of its span: the last such lane of each
+            // wave records its length
+            const int jn = __shfl_down(j, 1);
+            const int inn = __shfl_down((int)in, 1);
+            if (in && (lane == 63 || !inn || jn != j)) atomicMax(&sN[j], f - sS[j] + 1);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < WPT; ++q) {
+          count += __popc(bm[t + q * NT]);
+          bm[t + q * NT] = 0u;
+        }
+        // ---- cursors advance past the window; the next window starts at the smallest next column ----
+        if (t < 64) {
+          int nl = INT_MAX;
+          if (t < nA) {
+            const int took = sN[t];
+            nl = eNC[t];
+            if (took > 0) {
+              const int p = eP[t] + took;
+              eP[t] = p;
+              nl = p < eL[t] ? Bidx[eB0[t] + p] : INT_MAX;
+              eNC[t] = nl;
+            }
+          }
+          const int m = wave_reduce_dpp(nl, OpMin());
+          if (t == 0) s_lo = m;
+        }
+        __syncthreads();  // also: the bitmap is clear
+        lo = s_lo;
+        if (++windows > prow) {  // uniform: no progress; reported, the row's count is not trusted
+          if (t == 0) atomicOr(err, kErrWindow);
+          break;
+        }
+      }
+      count = Group<NT>::reduce(count, OpAdd(), scl);
+      if (t == 0) nnz[row] = count;
+      __syncthreads();
+      continue;
+    }
     int lo_l = INT_MAX;
     long long prow_l = 0;
     for (int64_t ka = a0 + t; ka < a1; ka += NT) {
@@ -1816,7 +1959,6 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 // LDS (sharing the word-prefix space) up to kOldStates entries, else in the workspace `state`
 // (16 bytes per A nonzero). Both paths leave the value array and the bitmap clear.
 constexpr int kNumLongStates = 512;
-constexpr int kSpanCap = 16;  // span-walk windows: elements per window of an entry without a band
 constexpr int kSpanMax = 256;  // span-walk windows: A entries per row at most (one per thread)
 
 
@@ -2954,7 +3096,8 @@ void launch_symbolic(const KernelShape& k, const int32_t* rows, const int64_t* p
   if (k.T == 0 || k.T == kHubShape) {  // the symbolic pass counts hub rows with the long-row windows
     const dim3 gr(grid_for(nrows_hint, gpb, grid));
     hipLaunchKernelGGL((k_symbolic_long<1024, kSymLongW, kLongG>), gr, dim3(1024), 0, s, rows, pbeg, pend, A.ptr,
-                       A.bstart, A.blen, B.idx, nnz, cursor, err, hub_slot, rng_cnt, nrng);
+                       A.bstart, A.blen, B.idx, nnz, cursor, err, hub_slot, rng_cnt, nrng, A.idx, B.hslot, B.band,
+                       B.nb1);
     return;
   }
 #define X(T_, NT_, G_)                                                                                         \
