@@ -174,6 +174,8 @@ def main():
                     help="er2m (default headline) or one of the other named configurations")
     ap.add_argument("--batch-members", type=int, default=512)
     ap.add_argument("--rmat-scale", type=int, default=22)
+    ap.add_argument("--table", type=int, default=None,
+                    help="rmat22 workload: run with this candidate category table (default: the default table)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations' lines (R-MAT 22 kernel window, forward+backward, "
                          "512-member batch) that the default run adds under \"extras\"")

@@ -175,10 +175,11 @@ def numeric_bytes(m, nnzA, P, nnzC, vsz=8):
     return 8 * (m + 1) + (4 + vsz) * nnzA + 8 * nnzA + (4 + vsz) * P + 8 * (m + 1) + (4 + vsz) * nnzC
 
 
-def rmat_kernel_window(scale=22, steps=1, warmup=1, A=None):
+def rmat_kernel_window(scale=22, steps=1, warmup=1, A=None, table=None):
     """R-MAT C = A*A, FP64, 512-row blocks, 64 GB device budget: the kernel window over the whole
     product (chunks streamed within the budget), with the numeric pass's roofline (algorithmic
-    bytes over the numeric pass's device time, summed over its chunks)."""
+    bytes over the numeric pass's device time, summed over its chunks). `table`: a candidate
+    category table id (default: the library's default table)."""
     import torch
 
     import dbd_amd
@@ -187,6 +188,8 @@ def rmat_kernel_window(scale=22, steps=1, warmup=1, A=None):
         A = dbd_amd.gen_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1)
     log(f"rmat{scale}: n={A.shape[0]} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
     ctx = dbd_amd.Context(block_rows=512, dtype=np.float64, device_gib=64, host_gib=256, retain=False)
+    if table is not None:
+        ctx.set_table(table)
     ctx.set_operands(A)
     info = ctx.plan()
     P = info["products"]
@@ -204,6 +207,8 @@ def rmat_kernel_window(scale=22, steps=1, warmup=1, A=None):
                                 "achieved": round(nb / num_s / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                                 "frac": round(nb / num_s / 1e9 / 8000.0, 4)},
            "num_rows_per_category": info["num_rows"]}
+    if table is not None:
+        res["table"] = table
     ctx.close()
     torch.cuda.empty_cache()
     log(f"rmat{scale}: kernel window {t * 1e3:.1f} ms ({2 * P / t / 1e9:.1f} GFLOPS), numeric pass "
@@ -249,7 +254,7 @@ def run_rmat22(args):
     import oracle
     torch.cuda.set_device(0)
     sc = args.rmat_scale
-    res, A = rmat_kernel_window(sc, max(1, args.steps), min(1, args.warmup))
+    res, A = rmat_kernel_window(sc, max(1, args.steps), min(1, args.warmup), table=args.table)
     P = res["products"]
     modes = [m for m in args.modes.split(",") if m]
     if "d2hfull" in modes:

@@ -139,7 +139,7 @@ constexpr int kKnobG[2] = {16, 64};
 constexpr int kKnobGBig[2] = {64, 16};
 constexpr double kKnobLoad[2] = {0.5, 0.25};
 constexpr int kKnobTop[2] = {8192, 4096};
-constexpr int64_t kKnobHub[3] = {kHubAutoCols, 0, INT64_MAX};  // by width, always, never
+constexpr int64_t kKnobHub[3] = {0, kHubWideCols, INT64_MAX};  // always, from 2^21 columns, never
 constexpr int kKnobRadix[5] = {2, 2, 2, 2, 3};
 
 std::vector<Tables>& candidate_tables() {
@@ -1250,13 +1250,18 @@ dbd_status dbd_backward(dbd_ctx* c, const int64_t* d_Cptr, const int32_t* d_Cidx
   const CatTable& nt = c->tb.num_ct;
   const int64_t* offs = c->offs_num.as<int64_t>();
   for (int k = 1; k < nt.n; ++k) {
-    const int64_t i0 = (int64_t)k * c->nwg, i1 = (int64_t)(k + 1) * c->nwg;
+    // long and hub rows both take the window backward kernel, which keeps a cursor per A entry;
+    // the two categories lie next to each other in the bins and share one launch, which takes
+    // its rows from the end (the hub rows, the heaviest, first)
+    const bool with_hub = c->tb.num_k[k].T == 0 && k + 1 < nt.n && c->tb.num_k[k + 1].T == kHubShape;
+    const int64_t i0 = (int64_t)k * c->nwg, i1 = (int64_t)(k + (with_hub ? 2 : 1)) * c->nwg;
     const int64_t n = c->h_offs_num[(size_t)i1] - c->h_offs_num[(size_t)i0];
+    const KernelShape ks = c->tb.num_k[k];
+    if (with_hub) ++k;
     if (n <= 0) continue;
-    // long and hub rows both take the window backward kernel, which keeps a cursor per A entry
-    if (c->tb.num_k[k].T == 0 || c->tb.num_k[k].T == kHubShape) CHK(ensure_cursor(c));
-    launch_backward(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
-                    d_Cptr, d_Cidx, d_dC, d_dA, d_dB, c->cursor.as<int64_t>(), c->err.as<int>(), s);
+    if (ks.T == 0 || ks.T == kHubShape) CHK(ensure_cursor(c));
+    launch_backward(c->o.dtype, ks, c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B, d_Cptr,
+                    d_Cidx, d_dC, d_dA, d_dB, c->cursor.as<int64_t>(), c->err.as<int>(), s);
   }
   return check_err(c, "backward");
 }

@@ -2931,8 +2931,9 @@ __global__ __launch_bounds__(NT) void k_backward_long(
   __shared__ long long scl[NT / 64];
   const int t = threadIdx.x;
   const int64_t beg = *pbeg, end = *pend;
-  for (int64_t slot = beg + blockIdx.x; slot < end; slot += gridDim.x) {
-    const int row = rows[slot];
+  // rows from the end of the range: callers put the heaviest rows last
+  for (int64_t s2 = beg + blockIdx.x; s2 < end; s2 += gridDim.x) {
+    const int row = rows[end - 1 - (s2 - beg)];
     const int64_t a0 = Aptr[row], a1 = Aptr[row + 1];
     const int64_t c0 = Cptr[row], c1 = Cptr[row + 1];
     for (int64_t ka = a0 + t; ka < a1; ka += NT) {

@@ -22,8 +22,9 @@
 
 namespace dbd {
 
-// Problems of at least this many columns route hub rows to the hub range kernels by default.
-constexpr int64_t kHubAutoCols = (int64_t)1 << 21;
+// The autotuner's alternative to the default (hub range kernels at every width): hub rows take
+// them from this many columns of B only.
+constexpr int64_t kHubWideCols = (int64_t)1 << 21;
 
 // A category table: thresholds and kernel shapes of the symbolic and numeric categories, and the
 // knobs they were built from (the autotuner's search space).
@@ -34,7 +35,7 @@ struct Tables {
   int g_small = 16, g_big = 64;  // lanes per A entry in tables of <= 1024 / larger slot counts
   double load = 0.5;             // hash-table load limit
   int top = 8192;                // slots of the largest numeric LDS table (symbolic: twice that)
-  int64_t hub_min_cols = kHubAutoCols;  // numeric hub category used when ncols >= this
+  int64_t hub_min_cols = 0;  // numeric hub category used when ncols >= this
 };
 
 // Symbolic key min(P_i, ncols), numeric key nnz_i; hash tables sized so the load stays <= `load`
@@ -42,7 +43,7 @@ struct Tables {
 // numeric tables stop at `top` slots (4096 or 8192), the symbolic ones at 2 * top. Rows past the
 // last table are long rows; the numeric table ends with the hub category, used when B has at
 // least hub_min_cols columns.
-Tables default_tables(int g_small, int g_big, double load, int top = 8192, int64_t hub_min_cols = kHubAutoCols);
+Tables default_tables(int g_small, int g_big, double load, int top = 8192, int64_t hub_min_cols = 0);
 
 // Whether the numeric binning routes hub rows (long rows of more than kHubEntries A entries) to
 // the hub category for a problem of ncols columns.

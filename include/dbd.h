@@ -200,8 +200,8 @@ typedef struct dbd_table_desc {
   int32_t top;          /* slots of the largest numeric LDS table (4096 or 8192; symbolic 2x);
                            rows beyond it take the long-row windows */
   int64_t hub_min_cols; /* long rows of more than 1024 A entries take the numeric hub range
-                           kernels when B has at least this many columns (2^21: by width; 0:
-                           always; INT64_MAX: never) */
+                           kernels when B has at least this many columns (0: always, the
+                           default; 2^21: wide problems only; INT64_MAX: never) */
 } dbd_table_desc;
 /* Candidate category tables (every combination of the knobs above: 48) the autotuner chooses
  * from; dbd_set_table selects one explicitly (results do not depend on the choice). */

@@ -70,7 +70,7 @@ def test_cli_usage_on_bad_argv():
 
 def test_candidate_tables_cover_every_knob_combination():
     """The autotuner's search space: 48 tables, every combination of the five knobs once, table 0
-    the default (16 / 64 lanes, load 0.5, 8192-slot largest table, hub kernels by width)."""
+    the default (16 / 64 lanes, load 0.5, 8192-slot largest table, hub kernels at every width)."""
     n = dbd_amd.Context.table_count()
     assert n == 48
     seen = set()
@@ -80,8 +80,8 @@ def test_candidate_tables_cover_every_knob_combination():
     assert len(seen) == 48
     assert {x[0] for x in seen} == {16, 64} and {x[1] for x in seen} == {16, 64}
     assert {x[2] for x in seen} == {0.5, 0.25} and {x[3] for x in seen} == {4096, 8192}
-    assert {x[4] for x in seen} == {1 << 21, 0, 2 ** 63 - 1}
+    assert {x[4] for x in seen} == {0, 1 << 21, 2 ** 63 - 1}
     assert dbd_amd.Context.table_knobs(0) == {"g_small": 16, "g_big": 64, "load": 0.5, "top": 8192,
-                                             "hub_min_cols": 1 << 21}
+                                             "hub_min_cols": 0}
     with pytest.raises(dbd_amd.DBDError):
         dbd_amd.Context.table_knobs(48)

@@ -98,9 +98,10 @@ def _skewed_matrix(n_rows, ncols, long_rows, seed, zipf=1.6):
 def test_all_categories(gpu_ok, dtype, table):
     """Row lengths from 1 to tens of thousands: every hash-table size and both window paths
     (symbolic key min(P_i, ncols) above 8192, numeric nnz_i above 4096) get rows. Table 0 (the
-    default) keeps this 60,000-column problem's hub rows in the long category; table 1 routes them
-    to the numeric hub range kernels (4 column ranges)."""
-    assert dbd_amd.Context.table_knobs(1)["hub_min_cols"] == 0
+    default) routes this 60,000-column problem's hub rows to the numeric hub range kernels (4
+    column ranges); table 1 (hub kernels from 2^21 columns only) keeps them in the long category."""
+    assert dbd_amd.Context.table_knobs(0)["hub_min_cols"] == 0
+    assert dbd_amd.Context.table_knobs(1)["hub_min_cols"] == 1 << 21
     n, ncols = 6000, 60000
     long_rows = {17: 3000, 900: 5000, 4500: 1200, 5999: 40}
     A = _skewed_matrix(n, n, long_rows, 11)
@@ -110,7 +111,7 @@ def test_all_categories(gpu_ok, dtype, table):
     # the symbolic pass's last category is the long rows; the numeric pass's the hub rows (long
     # rows of more than 1024 A entries: rows 17, 900 and 4500), the one before it the other long rows
     assert info["sym_rows"][-1] >= 4, info["sym_rows"]
-    if table == 1:
+    if table == 0:
         assert info["num_rows"][-1] >= 3, info["num_rows"]
     else:
         assert info["num_rows"][-1] == 0 and info["num_rows"][-2] >= 4, info["num_rows"]
