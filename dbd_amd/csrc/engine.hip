@@ -340,6 +340,8 @@ struct dbd_ctx {
   hipGraphExec_t gexec = nullptr;
   const void* g_key[3] = {nullptr, nullptr, nullptr};
   int64_t batch_cap = 0;  // entries of C the batch caller allocated (exact nnz at prepare)
+  // the batch's hub rows: position in the numeric bins, count, bucket capacity (from prepare)
+  int64_t bh_pos0 = 0, bh_n = 0, bh_nshort = 0;
   DevBuf g_rng;
 };
 
@@ -402,39 +404,69 @@ dbd_status assign_hub_slots(dbd_ctx* c) {
 
 // Hub rows rows_num[pos0, pos0 + n) of a chunk in column ranges: banded-entry lists and bucketed
 // short products, the symbolic pass's (row, range) column counts scanned into output offsets, the
-// numeric pass. One host synchronisation (the number of short products sizes their buckets).
-dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
-                       void* Cval) {
-  hipStream_t s = c->st;
-  const int nrng = hub_ranges(c->B.ncols);
-  const int64_t items = n * nrng;
-  const int32_t* rows = c->rows_num.as<int32_t>() + pos0;
+// numeric pass. hub_reserve sizes the workspace for n rows; hub_head lists the banded entries and
+// scans the short products' counts per (row, range); hub_tail fills the buckets (at most
+// `short_cap` products, else the capacity bit is set and nothing is written) and runs the ranges.
+dbd_status hub_reserve(dbd_ctx* c, int64_t n) {
+  const int64_t items = n * hub_ranges(c->B.ncols);
   CHK(c->hub_list.ensure(4 * (size_t)std::max<int64_t>(c->A.nnz, 1)));
   CHK(c->hub_cnt.ensure(4 * (size_t)n));
   CHK(c->hub_scnt.ensure(16 * (size_t)(items + 1)));  // counts, then their scan
   CHK(c->hub_rcnt.ensure(16 * (size_t)(items + 1)));
   CHK(c->hub_tmp.ensure(8 * (size_t)scan_tmp_elems(items + 1)));
+  return DBD_OK;
+}
+
+// The number of short products lands at hub_scnt[2 * items + 1] (the end of their scan).
+void hub_head(dbd_ctx* c, int64_t pos0, int64_t n, hipStream_t s) {
+  const int nrng = hub_ranges(c->B.ncols);
+  const int64_t items = n * nrng;
   int64_t* scnt = c->hub_scnt.as<int64_t>();
-  int64_t* soff = scnt + items + 1;
+  launch_hub_prepare(c->rows_num.as<int32_t>() + pos0, n, c->A, c->B, nrng, scnt, c->hub_list.as<int32_t>(),
+                     c->hub_cnt.as<int32_t>(), s);
+  launch_exclusive_scan(scnt, scnt + items + 1, items, c->hub_tmp.as<int64_t>(), s);
+}
+
+void hub_tail(dbd_ctx* c, int64_t pos0, int64_t n, int64_t short_cap, const int64_t* Cptr, int64_t cbase,
+              int32_t* Cidx, void* Cval, hipStream_t s) {
+  const int nrng = hub_ranges(c->B.ncols);
+  const int64_t items = n * nrng;
+  const int32_t* rows = c->rows_num.as<int32_t>() + pos0;
+  const int64_t* soff = c->hub_scnt.as<int64_t>() + items + 1;
   int64_t* rcnt = c->hub_rcnt.as<int64_t>();
   int64_t* roff = rcnt + items + 1;
-  launch_hub_prepare(rows, n, c->A, c->B, nrng, scnt, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(), s);
-  launch_exclusive_scan(scnt, soff, items, c->hub_tmp.as<int64_t>(), s);
-  int64_t nshort = 0;
-  HIPCHK(hipMemcpyAsync(&nshort, soff + items, 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  CHK(c->hub_scol.ensure(4 * (size_t)std::max<int64_t>(nshort, 1)));
-  CHK(c->hub_sval.ensure(c->vsz * (size_t)std::max<int64_t>(nshort, 1)));
-  launch_hub_scatter(c->o.dtype, rows, n, c->A, c->B, nrng, soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, s);
+  int* err = c->err.as<int>();
+  launch_hub_guard(soff + items, short_cap, err, s);
+  launch_hub_scatter(c->o.dtype, rows, n, c->A, c->B, nrng, soff, c->hub_scol.as<int32_t>(), c->hub_sval.p, err, s);
   launch_hub_gather_counts(rows, n, nrng, c->hub_slot.as<int32_t>(), c->hub_rng.as<int32_t>(), rcnt, s);
   launch_exclusive_scan(rcnt, roff, items, c->hub_tmp.as<int64_t>(), s);
   launch_hub_range(c->o.dtype, rows, n, nrng, c->A, c->B, c->hub_list.as<int32_t>(), c->hub_cnt.as<int32_t>(), soff,
-                   c->hub_scol.as<int32_t>(), c->hub_sval.p, roff, Cptr, cbase, Cidx, Cval, c->err.as<int>(), s);
+                   c->hub_scol.as<int32_t>(), c->hub_sval.p, roff, Cptr, cbase, Cidx, Cval, err, s);
+}
+
+// Sizes the buckets: one host synchronisation for the number of short products.
+dbd_status hub_buckets(dbd_ctx* c, int64_t n, int64_t* nshort) {
+  const int64_t items = n * hub_ranges(c->B.ncols);
+  *nshort = 0;
+  HIPCHK(hipMemcpyAsync(nshort, c->hub_scnt.as<int64_t>() + 2 * items + 1, 8, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  CHK(c->hub_scol.ensure(4 * (size_t)std::max<int64_t>(*nshort, 1)));
+  CHK(c->hub_sval.ensure(c->vsz * (size_t)std::max<int64_t>(*nshort, 1)));
+  return DBD_OK;
+}
+
+dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
+                       void* Cval) {
+  CHK(hub_reserve(c, n));
+  hub_head(c, pos0, n, c->st);
+  int64_t nshort = 0;
+  CHK(hub_buckets(c, n, &nshort));
+  hub_tail(c, pos0, n, nshort, Cptr, cbase, Cidx, Cval, c->st);
   return DBD_OK;
 }
 
 // Band index of B's long rows: the rows of at least `minlen` entries, minlen the smallest power
-// of two from 16 up whose rows' index fits min(1 GiB, device budget / 16). Built once per
+// of two from 16 up whose rows' index fits min(8 GiB, device budget / 4). Built once per
 // operand set (B does not change between runs).
 dbd_status build_band(dbd_ctx* c) {
   hipStream_t s = c->st;
@@ -448,7 +480,7 @@ dbd_status build_band(dbd_ctx* c) {
   HIPCHK(hipMemcpyAsync(hist, c->band_hist.p, sizeof(hist), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const int64_t nb1 = band_count(c->B.ncols);
-  const int64_t cap = std::min<int64_t>((int64_t)1 << 30, (c->o.device_budget_mib << 20) / 16);
+  const int64_t cap = std::min<int64_t>((int64_t)8 << 30, (c->o.device_budget_mib << 20) / 4);
   int j = 4;
   while (j < 31 && hist[j] * nb1 * 4 > cap) ++j;
   if (hist[j] == 0 || hist[j] * nb1 * 4 > cap) return DBD_OK;  // nothing to band: cursor walks only
@@ -625,6 +657,7 @@ dbd_status numeric_blocks(dbd_ctx* c, int64_t b0, int64_t b1, const int64_t* Cpt
     launch_numeric(c->o.dtype, c->tb.num_k[k], c->rows_num.as<int32_t>(), offs + i0, offs + i1, n, 0, c->A, c->B,
                    Cptr, cbase, Cidx, Cval, c->cursor.as<int64_t>(), c->err.as<int>(), c->st, distinct);
   }
+  HIPCHK(hipGetLastError());  // a launch the runtime refused
   return DBD_OK;
 }
 
@@ -1278,7 +1311,8 @@ static int persistent_grid(int num_cu, const KernelShape& k) {
 // persistent category kernels that read their row ranges from device memory: no host
 // synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
 // the numeric kernels drain empty ranges and the capacity bit is set in the error word. Hub rows
-// stay in the long category here (their range kernels size buckets on the host).
+// take their range kernels (w.hub_fn, buckets sized beforehand) when w.hub_slot is set, else they
+// stay in the long category.
 void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu,
                              int64_t* d_Cptr, int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s) {
   const int64_t nr = w.nr;
@@ -1292,18 +1326,20 @@ void dbd::enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B,
   launch_bin_scatter(w.P, nr, B.ncols, ct, w.offs_sym, w.rows_sym, s);
   for (int k = 1; k < ct.n; ++k)
     launch_symbolic(tb.sym_k[k], w.rows_sym, w.offs_sym + (int64_t)k * w.nwg, w.offs_sym + (int64_t)(k + 1) * w.nwg, 0,
-                    persistent_grid(num_cu, tb.sym_k[k]), A, B, w.nnz, w.cursor, w.err, s);
+                    persistent_grid(num_cu, tb.sym_k[k]), A, B, w.nnz, w.cursor, w.err, s, w.hub_slot, w.hub_rng,
+                    w.nrng);
   launch_exclusive_scan(w.nnz, d_Cptr, nr, w.tmp, s);
   const CatTable& nt = tb.num_ct;
-  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s);
+  launch_bin_count(w.nnz, nr, INT64_MAX, nt, w.cnt, s, w.hub_slot);
   launch_exclusive_scan(w.cnt, w.offs_num, (int64_t)nt.n * w.nwg, w.tmp, s);
-  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s);
+  launch_bin_scatter(w.nnz, nr, INT64_MAX, nt, w.offs_num, w.rows_num, s, w.hub_slot);
   launch_capacity_guard(d_Cptr + nr, cap, w.offs_num, (int64_t)nt.n * w.nwg + 1, w.err, s);
   for (int k = 1; k < nt.n; ++k) {
-    if (tb.num_k[k].T == kHubShape) continue;  // empty: no hub routing in the binning above
+    if (tb.num_k[k].T == kHubShape) continue;  // the range kernels below (empty without hub slots)
     launch_numeric(dtype, tb.num_k[k], w.rows_num, w.offs_num + (int64_t)k * w.nwg, w.offs_num + (int64_t)(k + 1) * w.nwg,
                    0, persistent_grid(num_cu, tb.num_k[k]), A, B, d_Cptr, 0, d_Cidx, d_Cval, w.cursor, w.err, s, false);
   }
+  if (w.hub_slot && w.hub_fn) w.hub_fn(w.hub_arg, d_Cptr, d_Cidx, d_Cval, s);
 }
 
 // The batch's pipeline on the context's workspace. C was sized by the symbolic pass of
@@ -1325,7 +1361,19 @@ static dbd_status batch_enqueue(dbd_ctx* c, hipStream_t s, int64_t* d_Cptr, int3
   w.err = c->err.as<int>();
   w.nr = c->nr;
   w.nwg = c->nwg;
+  if (c->bh_n > 0) {
+    w.hub_slot = c->hub_slot.as<int32_t>();
+    w.hub_rng = c->hub_rng.as<int32_t>();
+    w.nrng = hub_ranges(c->B.ncols);
+    w.hub_arg = c;
+    w.hub_fn = [](void* arg, const int64_t* Cptr, int32_t* Cidx, void* Cval, hipStream_t st) {
+      dbd_ctx* cc = static_cast<dbd_ctx*>(arg);
+      hub_head(cc, cc->bh_pos0, cc->bh_n, st);
+      hub_tail(cc, cc->bh_pos0, cc->bh_n, cc->bh_nshort, Cptr, 0, Cidx, Cval, st);
+    };
+  }
   enqueue_pipeline(w, c->tb, c->A, c->B, c->o.dtype, c->num_cu, d_Cptr, d_Cidx, d_Cval, c->batch_cap, s);
+  HIPCHK(hipGetLastError());  // a launch the runtime refused
   return DBD_OK;
 }
 
@@ -1341,6 +1389,20 @@ dbd_status dbd_batch_prepare(dbd_ctx* c, const dbd_csr* A_cat, const dbd_csr* B_
   // recomputes the same counts on every replay)
   CHK(symbolic(c));
   c->batch_cap = c->info.nnz;
+  // hub rows: their place in the numeric bins (the graph bins the same way) and the workspace of
+  // their range kernels, buckets included, sized here so the graph neither allocates nor waits
+  c->bh_n = c->bh_pos0 = c->bh_nshort = 0;
+  const CatTable& nt = c->tb.num_ct;
+  if (c->hub_n > 0 && c->tb.num_k[nt.n - 1].T == kHubShape) {
+    const int64_t p0 = c->h_offs_num[(size_t)(nt.n - 1) * c->nwg], p1 = c->h_offs_num[(size_t)nt.n * c->nwg];
+    if (p1 > p0) {
+      CHK(hub_reserve(c, p1 - p0));
+      hub_head(c, p0, p1 - p0, c->st);
+      CHK(hub_buckets(c, p1 - p0, &c->bh_nshort));
+      c->bh_pos0 = p0;
+      c->bh_n = p1 - p0;
+    }
+  }
   if (nnz_bound) *nnz_bound = c->info.nnz;
   return DBD_OK;
 }

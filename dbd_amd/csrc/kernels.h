@@ -136,7 +136,10 @@ void launch_hub_gather_counts(const int32_t* rows, int64_t n, int nrng, const in
 void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng, int64_t* scnt,
                         int32_t* hlist, int32_t* hcount, hipStream_t s);
 void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng,
-                        const int64_t* soff, int32_t* scol, void* sval, hipStream_t s);
+                        const int64_t* soff, int32_t* scol, void* sval, const int* err, hipStream_t s);
+// Sets the capacity bit in err when *total (the hub rows' short products) exceeds cap: the hub
+// scatter and range kernels then write nothing.
+void launch_hub_guard(const int64_t* total, int64_t cap, int* err, hipStream_t s);
 void launch_hub_range(int dtype, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
                       const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
                       const void* sval, const int64_t* roff, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,

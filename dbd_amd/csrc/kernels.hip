@@ -1075,16 +1075,15 @@ __device__ __forceinline__ int long_lanes(int64_t nA) {
 
 // Span walks (long-row windows, symbolic and numeric): elements per window of an A entry whose B
 // row has no band index entry; the window ends before the first element such a span leaves out.
-constexpr int kSpanCap = 16;
-// The symbolic windows are 32 times wider than the numeric dense windows: a cap of 64 elements
-// (no fewer than the band index's smallest banded length on large problems) keeps them whole.
-constexpr int kSymSpanCap = 64;
+// 64 (measured against 16 and 256): B rows shorter than the band index's smallest banded length
+// rarely cut a window, and a span re-reads at most 64 elements.
+constexpr int kSpanCap = 64;
 constexpr int kSymSpanEntries = 64;  // symbolic span walks: rows of up to one wave of entries
 
 // Rows too long for an LDS table: sweep the row's columns in windows of W bits.
 // With B's band index, rows of up to kSymSpanEntries entries (no hub slot) walk
 // element-parallel: every active entry's element span in the window (banded B rows: up to the
-// band boundary at or past the window end; other B rows: at most kSymSpanCap elements, the
+// band boundary at or past the window end; other B rows: at most kSpanCap elements, the
 // window then ending before the first element left out) is flattened by a scan in wave 0, each
 // thread of the workgroup takes U elements at a time, and the entry state (B row start and
 // length, elements consumed, next column, band slot) lives in LDS. (A workgroup-wide version for
@@ -1162,7 +1161,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
             if (sl >= 0) {
               q = band[(int64_t)sl * nb1 + min((hi0 + (1 << kBandShift) - 1) >> kBandShift, nb1 - 1)];
             } else {
-              q = min(L, p + kSymSpanCap);
+              q = min(L, p + kSpanCap);
               if (q < L) clip = Bidx[eB0[t] + q];
             }
             len = q - p;
@@ -1404,8 +1403,9 @@ __global__ __launch_bounds__(NT) void k_hub_scatter(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ Bidx, const V* __restrict__ Bval,
                                                     const int32_t* __restrict__ hslot, int nrng,
                                                     const int64_t* __restrict__ soff, int32_t* __restrict__ scol,
-                                                    V* __restrict__ sval) {
+                                                    V* __restrict__ sval, const int* __restrict__ err) {
   extern __shared__ unsigned long long lcur[];  // nrng write positions
+  if (*err & kErrCapacity) return;  // the buckets (or C) are smaller than this structure needs
   const int t = threadIdx.x;
   const int64_t j = blockIdx.x;
   const int row = rows[j];
@@ -1448,8 +1448,11 @@ __global__ __launch_bounds__(NT) void k_hub_range(
   __shared__ V hva[NT];
   __shared__ int hS[NT];
   __shared__ int sc[NW];
+  if (*err & kErrCapacity) return;  // the buckets or C are smaller than this structure needs
   const int t = threadIdx.x;
   const int64_t item = blockIdx.x;
+  // a range without columns (by the symbolic count) holds no product: nothing to add or emit
+  if (roff[item + 1] == roff[item]) return;
   const int64_t j = item / nrng;
   const int r = (int)(item - j * nrng);
   const int row = rows[j];
@@ -3219,27 +3222,45 @@ void launch_hub_prepare(const int32_t* rows, int64_t nh, const Mat& A, const Mat
 }
 
 void launch_hub_scatter(int dtype, const int32_t* rows, int64_t nh, const Mat& A, const Mat& B, int nrng,
-                        const int64_t* soff, int32_t* scol, void* sval, hipStream_t s) {
+                        const int64_t* soff, int32_t* scol, void* sval, const int* err, hipStream_t s) {
   if (nh <= 0) return;
   const size_t lds = (size_t)nrng * sizeof(unsigned long long);
   if (dtype == 1)
     hipLaunchKernelGGL((k_hub_scatter<double, 256>), dim3((unsigned)nh), dim3(256), lds, s, rows, A.ptr, A.idx,
                        A.bstart, A.blen, (const double*)A.val, B.idx, (const double*)B.val, B.hslot, nrng, soff, scol,
-                       (double*)sval);
+                       (double*)sval, err);
   else
     hipLaunchKernelGGL((k_hub_scatter<float, 256>), dim3((unsigned)nh), dim3(256), lds, s, rows, A.ptr, A.idx,
                        A.bstart, A.blen, (const float*)A.val, B.idx, (const float*)B.val, B.hslot, nrng, soff, scol,
-                       (float*)sval);
+                       (float*)sval, err);
 }
 
+// Bucket capacity check of the hub path inside a captured pipeline: more short products than the
+// buckets hold flags the capacity bit, on which the scatter and range kernels write nothing.
+__global__ void k_hub_guard(const int64_t* __restrict__ total, int64_t cap, int* __restrict__ err) {
+  if (*total > cap) atomicOr(err, kErrCapacity);
+}
+
+void launch_hub_guard(const int64_t* total, int64_t cap, int* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_hub_guard, dim3(1), dim3(1), 0, s, total, cap, err);
+}
+
+// One workgroup per (row, range) item; a launch holds fewer than 2^32 work-items, so rows go in
+// groups of at most (2^32 / 1024 - 1) / nrng, each launch with its rows' slices of the per-row
+// and per-item arrays.
 template <typename V>
 static void launch_hub_range_t(const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,
                                const int32_t* hlist, const int32_t* hcount, const int64_t* soff, const int32_t* scol,
                                const V* sval, const int64_t* roff, const int64_t* Cptr, int64_t cbase, int32_t* Cidx,
                                V* Cval, int* err, hipStream_t s) {
-  hipLaunchKernelGGL((k_hub_range<V, 1024>), dim3((unsigned)(nh * nrng)), dim3(1024), 0, s, rows, nrng, A.ptr, A.idx,
-                     A.bstart, (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist, hcount, soff,
-                     scol, sval, roff, Cptr, cbase, Cidx, Cval, err);
+  constexpr int64_t kMaxItems = ((int64_t)1 << 32) / 1024 - 1;
+  const int64_t per = std::max<int64_t>(1, kMaxItems / nrng);
+  for (int64_t j0 = 0; j0 < nh; j0 += per) {
+    const int64_t nj = std::min(per, nh - j0), i0 = j0 * nrng;
+    hipLaunchKernelGGL((k_hub_range<V, 1024>), dim3((unsigned)(nj * nrng)), dim3(1024), 0, s, rows + j0, nrng, A.ptr,
+                       A.idx, A.bstart, (const V*)A.val, B.idx, (const V*)B.val, B.hslot, B.band, B.nb1, hlist,
+                       hcount + j0, soff + i0, scol, sval, roff + i0, Cptr, cbase, Cidx, Cval, err);
+  }
 }
 
 void launch_hub_range(int dtype, const int32_t* rows, int64_t nh, int nrng, const Mat& A, const Mat& B,

@@ -57,13 +57,20 @@ struct Work {
   int32_t *rows_sym, *rows_num, *blen;
   int* err;
   int64_t nr, nwg;
+  // hub rows (all null / 0: none): their slots and per-range column counts for the symbolic pass
+  // and the numeric binning, and the enqueue of their range kernels after the numeric categories
+  const int32_t* hub_slot = nullptr;
+  int32_t* hub_rng = nullptr;
+  int nrng = 0;
+  void (*hub_fn)(void* arg, const int64_t* Cptr, int32_t* Cidx, void* Cval, hipStream_t s) = nullptr;
+  void* hub_arg = nullptr;
 };
 
 // The whole pipeline (row analysis, bins, symbolic, scan, bins, numeric) on stream s with
 // persistent category kernels that read their row ranges from device memory: no host
 // synchronisation and no allocation, so it can be captured in a graph. If nnz(C) exceeds `cap`,
 // the numeric kernels drain empty ranges and the capacity bit is set in the error word.
-// Hub rows stay in the long category here.
+// Without w.hub_slot, hub rows stay in the long category.
 void enqueue_pipeline(const Work& w, const Tables& tb, Mat A, const Mat& B, int dtype, int num_cu, int64_t* d_Cptr,
                       int32_t* d_Cidx, void* d_Cval, int64_t cap, hipStream_t s);
 

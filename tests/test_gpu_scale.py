@@ -91,6 +91,36 @@ def test_rmat22_hub_rows_and_trailing_rows_products_above_2_31(gpu_ok):
     compare_rows(rows, C.ptr, C.idx, C.val, A, A, np.float64, "R-MAT 22 slice")
 
 
+def test_hub_rows_beyond_one_launch_of_range_workgroups(gpu_ok):
+    """1100 hub rows of 1030 A entries over 4096 column ranges (B: 5000 rows of 64 entries spread
+    over 67,108,864 columns): 4.5 million (row, range) workgroups of 1024 threads, more than one
+    launch can hold (fewer than 2^32 work-items), so the range kernel goes out in row groups.
+    Sampled rows against the oracle; every row length against its product count."""
+    rng = np.random.default_rng(77)
+    ncols, nb, lb, nh, la = 4096 * 16384, 5000, 64, 1100, 1030
+    bidx = np.empty((nb, lb), np.int32)
+    for i in range(nb):
+        bidx[i] = np.unique(rng.integers(0, ncols, 2 * lb))[:lb]
+    B = dbd_amd.CSR(np.arange(0, nb * lb + 1, lb, dtype=np.int64), bidx.ravel(), rng.standard_normal(nb * lb),
+                    (nb, ncols))
+    aidx = np.stack([np.sort(rng.choice(nb, la, replace=False)) for _ in range(nh)]).astype(np.int32)
+    A = dbd_amd.CSR(np.arange(0, nh * la + 1, la, dtype=np.int64), aidx.ravel(), rng.standard_normal(nh * la),
+                    (nh, nb))
+    with dbd_amd.Context(block_rows=512, dtype=np.float64) as ctx:
+        ctx.set_operands(A, B)
+        ctx.run("compute")
+        info = ctx.plan_info()
+        assert info["num_rows"][-1] == nh, info["num_rows"]  # every row in the hub category
+        C = ctx.result()
+    assert C.ptr[0] == 0 and C.ptr[-1] == C.nnz and np.all(np.diff(C.ptr) > 4096)
+    assert np.all(np.diff(C.ptr) <= la * lb)
+    sel = np.sort(rng.choice(nh, 24, replace=False)).astype(np.int32)
+    lens = C.ptr[sel + 1] - C.ptr[sel]
+    sptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    take = np.concatenate([np.arange(C.ptr[r], C.ptr[r + 1]) for r in sel])
+    compare_rows(sel, sptr, C.idx[take], C.val[take], A, B, np.float64, "hub rows in row groups")
+
+
 def test_er2m_33_per_row_files_nnz_above_2_31(gpu_ok):
     n, deg = 2_000_000, 33
     A = dbd_amd.gen_er(n, n, deg, seed=31)
