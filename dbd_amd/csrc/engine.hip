@@ -466,7 +466,7 @@ dbd_status hub_numeric(dbd_ctx* c, int64_t pos0, int64_t n, const int64_t* Cptr,
 }
 
 // Band index of B's long rows: the rows of at least `minlen` entries, minlen the smallest power
-// of two from 16 up whose rows' index fits min(8 GiB, device budget / 4). Built once per
+// of two from 64 up whose rows' index fits min(8 GiB, device budget / 4). Built once per
 // operand set (B does not change between runs).
 dbd_status build_band(dbd_ctx* c) {
   hipStream_t s = c->st;
@@ -481,7 +481,9 @@ dbd_status build_band(dbd_ctx* c) {
   HIPCHK(hipStreamSynchronize(s));
   const int64_t nb1 = band_count(c->B.ncols);
   const int64_t cap = std::min<int64_t>((int64_t)8 << 30, (c->o.device_budget_mib << 20) / 4);
-  int j = 4;
+  // from 64 up: shorter B rows are read whole by the span walks (at most 64 elements a window),
+  // and leaving them out of the index saves the hub kernels their per-range band reads
+  int j = 6;
   while (j < 31 && hist[j] * nb1 * 4 > cap) ++j;
   if (hist[j] == 0 || hist[j] * nb1 * 4 > cap) return DBD_OK;  // nothing to band: cursor walks only
   const int64_t H = hist[j];
