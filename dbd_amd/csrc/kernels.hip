@@ -1939,23 +1939,26 @@ __global__ __launch_bounds__(NT) void k_numeric_block(DBD_NUMERIC_ARGS) {
 
 // Long rows, any column range: the row's columns are swept in windows [lo, hi); a window starts
 // at the smallest next column over the row's A entries, so empty column ranges are skipped, and
-// every window's output is produced in column order (no sort). Each A entry is walked by a
-// subgroup of G lanes (long_lanes) from a cursor (LongState: position, remaining, next column).
+// every window's output is produced in column order (no sort).
 //
-// Rows whose entries each have a subgroup (nA <= NT/G) take windows of up to BW columns, with
-// two walks over the window's products:
-//   1. presence: every B column in the window sets its bit in an LDS bitmap; each A entry
-//      counts its elements in the window (U loads per lane in flight);
+// Rows of up to kSpanMax entries (with B's band index; without it, rows whose entries each have
+// a subgroup of G lanes, nA <= NT/G) take windows of up to BW columns, with two walks over the
+// window's products:
+//   1. presence: every B column in the window sets its bit in an LDS bitmap (span walks: every
+//      thread takes U elements of the flattened spans per step; cursor walks: each A entry's
+//      subgroup counts its elements in the window, U loads per lane in flight);
 //   2. values: the bitmap's word prefixes give every column its rank among the window's
 //      columns, and A_ik * B_kj is added at that rank in an LDS value array of VC entries.
 // The values are then stored as they lie (coalesced) and the columns read off the bitmap. When
 // more than VC columns are present the window is cut at the VC-th column (the values walk stops
 // there, the next window starts at it) and the next window is narrowed to the width that held
-// VC columns; windows widen again (doubling up to BW) when they fit. Between the two walks an
-// active entry keeps -1 - count in its `nc` field.
+// VC columns; windows widen again (doubling up to BW) when they fit. Between the two cursor walks
+// an active entry keeps -1 - count in its `nc` field.
 //
 // Rows of more entries take dense windows of VC columns in one walk (values added at column -
-// lo, presence bits beside them), 4 lanes per entry (8 when an entry averages 4 or more elements
+// lo, presence bits beside them): banded windows (below) when their entries fit the registers,
+// else cursor windows, each A entry walked by a subgroup from a cursor (LongState: position,
+// remaining, next column), 4 lanes per entry (8 when an entry averages 4 or more elements
 // per window over the row's column span), 4 entries per subgroup in flight, a B row with more elements
 // in the window than one step continuing UL steps at a time (loads issued together); an entry
 // whose next column lies beyond the window costs one state read per window. Their cursors are in
