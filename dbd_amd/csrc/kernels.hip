@@ -3197,7 +3197,9 @@ __global__ void k_hub_gather_counts(const int32_t* __restrict__ rows, int64_t n,
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < items; q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = q / nrng;
     const int r = (int)(q - j * nrng);
-    rcnt[q] = rng_cnt[(int64_t)hub_slot[rows[j]] * nrng + r];
+    // a row without a slot (only if a captured pipeline's operands changed structure) has none
+    const int hs = hub_slot[rows[j]];
+    rcnt[q] = hs >= 0 ? rng_cnt[(int64_t)hs * nrng + r] : 0;
   }
 }
 
