@@ -1449,7 +1449,7 @@ __global__ __launch_bounds__(NT) void k_hub_range(
   __shared__ int hS[NT];
   __shared__ int sc[NW];
   if (*err & kErrCapacity) return;  // the buckets or C are smaller than this structure needs
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
   const int64_t item = blockIdx.x;
   // a range without columns (by the symbolic count) holds no product: nothing to add or emit
   if (roff[item + 1] == roff[item]) return;
@@ -1464,45 +1464,53 @@ __global__ __launch_bounds__(NT) void k_hub_range(
   for (int q = t; q < NWORD; q += NT) bm[q] = 0u;
   for (int q = t; q < kHubRange; q += NT) dv[q] = V(0);
   __syncthreads();
-  // products of the banded entries in this range, element-parallel over the workgroup
+  // products of the banded entries in this range: chunks of NT entries, each wave taking 64 of
+  // them (strided, so the heaviest entries, which lie together in A's order, spread over the
+  // waves) and gathering its entries' spans element-parallel on its own: a wave scan flattens
+  // them and an element's owner is searched among 64 entries, with no workgroup barrier
+  const int wv = t >> 6;
+  int* whS = hS + (t & ~63);
+  const int64_t* whb0 = hb0 + (t & ~63);
+  const V* whva = hva + (t & ~63);
   for (int e0 = 0; e0 < nH; e0 += NT) {  // uniform
+    const int e = e0 + wv + (NT / 64) * lane;
     int len = 0;
-    if (e0 + t < nH) {
-      const int64_t ka = a0 + hlist[a0 + e0 + t];
+    if (e < nH) {
+      const int64_t ka = a0 + hlist[a0 + e];
       const int32_t* bd = band + (int64_t)hslot[Aidx[ka]] * nb1;
       const int s0 = bd[w0];
       len = bd[w1] - s0;
       hb0[t] = bstart[ka] + s0;
       hva[t] = Aval[ka];
     }
-    int T;
-    hS[t] = Group<NT>::exscan(len, T, sc);
-    const int nch = min(NT, nH - e0);
-    __syncthreads();
-    for (int f0 = 0; f0 < T; f0 += NT * U) {  // uniform
+    const int incl = wave_incl_scan_dpp(len);
+    hS[t] = incl - len;  // lanes past the row's entries hold the wave total: never an owner
+    const int T = __builtin_amdgcn_readlane(incl, 63);
+    wave_sync();
+    for (int f0 = 0; f0 < T; f0 += 64 * U) {  // wave-uniform
       int col[U];
       V bv[U], va[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int f = f0 + u * NT + t;
-        int jj = 0;  // largest jj with hS[jj] <= f
+        const int f = f0 + u * 64 + lane;
+        int jj = 0;  // largest jj with whS[jj] <= f
 #pragma unroll
-        for (int step = NT / 2; step >= 1; step >>= 1)
-          if (jj + step < nch && hS[jj + step] <= f) jj += step;
-        const int64_t at = f < T ? hb0[jj] + (f - hS[jj]) : 0;  // lanes past T: unused read of B's first entry
+        for (int step = 32; step >= 1; step >>= 1)
+          if (whS[jj + step] <= f) jj += step;  // jj + step <= 63
+        const int64_t at = f < T ? whb0[jj] + (f - whS[jj]) : 0;  // lanes past T: unused read of B's first entry
         col[u] = Bidx[at];
         bv[u] = Bval[at];
-        va[u] = hva[jj];
+        va[u] = whva[jj];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (f0 + u * NT + t < T) {
+        if (f0 + u * 64 + lane < T) {
           const int o = (int)(col[u] - c0);
           atomicOr(&bm[o >> 5], 1u << (o & 31));
           lds_add(&dv[o], va[u] * bv[u]);
         }
     }
-    __syncthreads();  // the chunk tables are rewritten by the next chunk
+    wave_sync();  // this wave's tables are rewritten by its next chunk
   }
   // products of the short entries, from the range's bucket
   const int64_t q0 = soff[item], q1 = soff[item + 1];
