@@ -914,7 +914,7 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
     HIPCHK(hipEventRecord(c->ev_copy[bi], c->cst));
     if (mode == DBD_MODE_IO) {
       HIPCHK(hipEventRecord(c->ev_slot[slot], c->cst));
-      wr.push({slot, c->ev_slot[slot], hi, hv, n, ch.e0});
+      wr.push({slot, c->ev_slot[slot], hi, hv, n, ch.e0, {}});
     }
   }
   HIPCHK(hipEventRecord(c->ev_t1, s));
@@ -925,7 +925,8 @@ dbd_status run(dbd_ctx* c, int mode, dbd_timing* tm) {
   HIPCHK(hipStreamSynchronize(s));
   if (mode == DBD_MODE_IO) {
     wr.finish();
-    const bool closed_ok = (close(wr.fd_idx) == 0) & (close(wr.fd_val) == 0);
+    const int ci = close(wr.fd_idx), cv = close(wr.fd_val);  // both, whatever the first returns
+    const bool closed_ok = ci == 0 && cv == 0;
     wr.fd_idx = wr.fd_val = -1;
     if (!closed_ok) return set_error(DBD_ERR_IO, "closing the output files failed: %s", strerror(errno));
     if (wr.failed) return set_error(DBD_ERR_IO, "Error during File Write: %s", wr.why.c_str());

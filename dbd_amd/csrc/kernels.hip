@@ -919,7 +919,7 @@ __global__ __launch_bounds__(kBinTile) void k_bin_count(const int64_t* __restric
     atomicAdd(&hist[row_category(k, ct, hub_slot, i)], 1);
   }
   __syncthreads();
-  if (threadIdx.x < ct.n) counts[(int64_t)threadIdx.x * nwg + blockIdx.x] = hist[threadIdx.x];
+  if ((int)threadIdx.x < ct.n) counts[(int64_t)threadIdx.x * nwg + blockIdx.x] = hist[threadIdx.x];
 }
 
 __global__ __launch_bounds__(kBinTile) void k_bin_scatter(const int64_t* __restrict__ key, int64_t n, int64_t cap,
